@@ -1,0 +1,212 @@
+/*
+ * libspacetime_hip — C ABI of the MI355X (gfx950) Kerr–Schild null-geodesic
+ * tracer.  Plain C types only: host pointers, sizes and small parameter
+ * blocks.  Device memory is owned by the library inside an opaque context.
+ *
+ * Each entry point names the reference interface it replaces (SpaceTime.jl,
+ * file:line).  Layouts follow the reference's Julia column-major arrays, so
+ * a Julia caller passes its arrays without repacking:
+ *   Julia (3, W, H) Float32      == C float[H][W][3]
+ *   Julia (nr, nphi, nz) Float32 == C float[nz][nphi][nr]
+ *   Julia (3, N) Float32 LUT     == C float[N][3]
+ * Parameter blocks are 0-based copies of the reference's 1-based vectors:
+ *   cam_params[49]       src/metal.jl:1713-1774
+ *   spacetime_params[8]  src/metal.jl:1850-1854  (M, r_band, r_escape,
+ *                        relativistic, h_coef, h_cap, a, kill radius^2)
+ *   disc_params[6]       src/metal.jl:146-148   (inner, outer, falloff,
+ *                        LUT T_min, T_max, size)
+ *   vol_params[9]        src/metal.jl:162-165
+ *   star_params[16]      src/metal.jl:286-290
+ *
+ * Errors: every call returns ST_OK (0) or an ST_ERR_* code; st_last_error()
+ * gives a thread-local message.  Non-finite rays are painted black, never an
+ * error (src/metal.jl:1414-1421).
+ * Threading: a context carries its own HIP stream; calls on one context must
+ * not overlap from several host threads.
+ */
+#ifndef SPACETIME_HIP_H
+#define SPACETIME_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ST_OK = 0,
+    ST_ERR_INVALID_ARG = 1,
+    ST_ERR_OOM = 2,
+    ST_ERR_DEVICE = 3,
+    ST_ERR_LAUNCH = 4,
+    ST_ERR_UNSUPPORTED = 5
+};
+
+enum { ST_CAM_PARAMS_N = 49, ST_SPACETIME_PARAMS_N = 8, ST_DISC_PARAMS_N = 6,
+       ST_VOL_PARAMS_N = 9, ST_STAR_PARAMS_N = 16 };
+
+/* Precision policy of a trace. */
+enum {
+    ST_PREC_FP32 = 0,   /* every ray in FP32 */
+    ST_PREC_MIXED = 1,  /* FP32 bulk; photon-shell grazing and near-horizon rays
+                           re-traced in FP64 from a compacted queue */
+    ST_PREC_FP64 = 2    /* every ray in FP64 */
+};
+
+/* Per-ray record layout (ST_REC_N floats per pixel) written when records
+ * are enabled on the context. */
+enum {
+    ST_REC_FATE = 0,    /* 0 escaped, 1 captured, 2 non-finite */
+    ST_REC_STEPS = 1,
+    ST_REC_DIR = 2,     /* 3 floats: unit escape direction */
+    ST_REC_RGB = 5,     /* 3 floats: this sample's radiance */
+    ST_REC_HAM = 8,     /* H / p_t^2 at termination */
+    ST_REC_REND = 9,    /* |x| at termination */
+    ST_REC_FLAGS = 10,  /* bit0 traced in FP64, bit1 Bardeen captured, bit2 graze */
+    ST_REC_RKMIN = 11,  /* minimum KS r along the path */
+    ST_REC_N = 12
+};
+
+typedef struct st_ctx st_ctx;
+
+/* Scalars of one trace launch: the keyword arguments of the reference's
+ * _launch_trace! (src/metal.jl:1888-1897) plus this library's step-rule and
+ * precision extensions.  st_trace_args_default() fills the reference's
+ * defaults. */
+typedef struct {
+    int width, height;     /* sensor size (pixels) */
+    int nmax;              /* step budget per ray */
+    float dt;              /* base affine step */
+    float tol;             /* adaptive tolerance (reserved for ORD 45/46) */
+    float ju, jv;          /* sub-pixel sample point, or stratum origin if jw > 0 */
+    float jw;              /* per-pixel jitter width (0 = pass-wide point) */
+    int jseed;             /* per-pixel jitter hash seed */
+    float weight;          /* out += weight * sample */
+    int row0, rows;        /* row tile [row0, row0 + rows) */
+    int col0, cols;        /* column tile; cols < 0 means the full width */
+    int order;             /* 4 RK4, 2 midpoint, 1 Euler */
+    /* extensions (no counterpart in the reference's launch) */
+    float h_floor;         /* lower clamp of h_coef * r / M (reference: 1) */
+    int capture_mode;      /* 0 reference kill rules, 1 KS r < capture_factor * r_plus */
+    float capture_factor;
+    int kerr_shrink;       /* reference step shrink near the Kerr kill radius */
+    int esc_interp;        /* escape direction interpolated to |x| = r_escape */
+    int bardeen;           /* Kerr launch-time capture verdict forces shadow */
+    int precision;         /* ST_PREC_* */
+    float promote_band;    /* FP64 promotion: |r_peri - r_ph| < band (M units) */
+    float promote_horizon; /* FP64 promotion: KS r < promote_horizon * r_plus */
+    int stat;              /* reserved: refinement statistics variant */
+} st_trace_args;
+
+/* Counters of the last st_trace / st_trace_batch on a context. */
+typedef struct {
+    uint64_t rays;          /* rays launched */
+    uint64_t steps_fp32;    /* integrator steps taken in FP32 */
+    uint64_t steps_fp64;    /* integrator steps taken in FP64 */
+    uint64_t promoted;      /* rays re-traced in FP64 */
+    float ms_fp32;          /* device time of the FP32 (or only) kernel */
+    float ms_fp64;          /* device time of the FP64 queue kernel */
+    float ms_total;         /* device time of the whole trace call */
+} st_stats;
+
+/* Defaults of the reference's render_draft_mtl / _launch_trace! keywords. */
+void st_trace_args_default(st_trace_args* args, int width, int height);
+
+/* Context.  Replaces MetalPreviewContext(background, width, height; ...)
+ * (src/metal.jl:129-182) and _upload_background (src/metal.jl:111-121).
+ * bg: float[bg_h][bg_w][3] equirectangular sky (row = theta, column = phi). */
+int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, int height);
+int st_ctx_destroy(st_ctx* ctx);
+
+/* set_disc_enabled! / disc upload (src/metal.jl:140-154, 206-214).
+ * lut: float[n][3]; disc_params[0] = 0 disables the disc. */
+int st_ctx_set_disc(st_ctx* ctx, const float disc_params[6], const float* lut, int n);
+/* Temperature law of the disc and gas: T = T0 * (s_T / r0)^(-3/4) with
+ * s_T = max(s, r_isco) when hold_isco (the reference's law,
+ * src/metal.jl:1528-1531: T0 = exp(10.034259), r0 = 2M, hold on, plus the
+ * (s/r_isco)^3 plunge taper).  No counterpart in the reference API. */
+int st_ctx_set_disc_temperature(st_ctx* ctx, float T0, float r0, int hold_isco);
+/* Volumetric disc grid (src/metal.jl:155-166).  vol: float[nz][nphi][nr]. */
+int st_ctx_set_volume(st_ctx* ctx, const float* vol, int nr, int nphi, int nz,
+                      const float vol_params[9]);
+/* set_volume_enabled! (src/metal.jl:192-196) */
+int st_ctx_set_volume_enabled(st_ctx* ctx, int enabled);
+/* set_march_stride! (src/metal.jl:304-308) */
+int st_ctx_set_march_stride(st_ctx* ctx, int stride);
+/* set_starfield! (src/metal.jl:257-295).  lut: float[n][3] star colour LUT. */
+int st_ctx_set_starfield(st_ctx* ctx, const float star_params[16], const float* lut, int n);
+/* Per-pixel records (ST_REC_N floats per pixel) on/off; allocated lazily. */
+int st_ctx_enable_records(st_ctx* ctx, int on);
+
+/* Zero the context's (3, W, H) accumulation buffer (fill!(out, 0),
+ * src/metal.jl:1857). */
+int st_clear(st_ctx* ctx);
+
+/* One trace dispatch: _launch_trace! (src/metal.jl:1888-1933) into the
+ * context's buffer.  cam_params64 (optional, may be NULL) is the same block
+ * in double precision for the FP64 path.  Asynchronous on the context's
+ * stream. */
+int st_trace(st_ctx* ctx, const float cam_params[49], const double* cam_params64,
+             const float spacetime_params[8], const st_trace_args* args);
+
+/* Wait for the context's stream (Metal.synchronize(), src/metal.jl:2180). */
+int st_sync(st_ctx* ctx);
+/* Copy the (3, W, H) buffer to host float[H][W][3] (_download_rgb,
+ * src/metal.jl:1991-1998; copyto!(host, out_gpu), src/metal.jl:1813). */
+int st_download(st_ctx* ctx, float* host_out);
+/* Copy the per-pixel records to host float[H][W][ST_REC_N]. */
+int st_download_records(st_ctx* ctx, float* host_rec);
+/* Counters and device timings of the last trace call (synchronises). */
+int st_get_stats(st_ctx* ctx, st_stats* out);
+
+/* ---- batched tracing: many cameras / frames of uneven size in one launch ----
+ * No counterpart in the reference (it loops _launch_trace! per frame,
+ * src/metal.jl:2131-2189).  Frame k is traced with cam_params[k*49 ...] into
+ * out + out_offset[k] (float[h_k][w_k][3], assigned, not accumulated); the
+ * whole set is one persistent launch over a work queue of pixel tiles. */
+typedef struct {
+    int width, height;     /* frame size */
+    int out_offset;        /* float offset of the frame in the batch output */
+} st_frame_desc;
+
+int st_batch_create(st_ctx* ctx, int nframes, const st_frame_desc* frames,
+                    const float* cam_params /* [nframes][49] */,
+                    const double* cam_params64 /* may be NULL */,
+                    const float spacetime_params[8]);
+/* Trace every frame of the batch; tile/waves are the launch shape (0 = the
+ * autotuned choice for this device). */
+int st_trace_batch(st_ctx* ctx, const st_trace_args* args, int tile, int waves_per_cu);
+/* Same, captured once into a hipGraph and replayed on later calls. */
+int st_trace_batch_graph(st_ctx* ctx, const st_trace_args* args, int tile, int waves_per_cu);
+int st_download_batch(st_ctx* ctx, float* host_out, size_t nfloats);
+
+/* ---- host helpers (FP64 host math, no device work) ---- */
+/* _ks_cam_params (src/metal.jl:1760-1774) for Camera(pos, target, up, fov;
+ * velocity) (src/camera.jl:44-49), tetrad from ks_camera_tetrad
+ * (src/preview.jl:93-198).  Writes 49 doubles. */
+int st_camera_params(const double pos[3], const double target[3], const double up[3],
+                     double fov_factor, const double velocity[3], double M, double a,
+                     double fisheye_deg, double focus_dist, int equirect, double out[49]);
+/* Same from an explicit basis (fwd, right, up) and camera-frame beta. */
+int st_camera_params_basis(const double pos[3], const double fwd[3], const double right[3],
+                           const double up[3], double fov_factor, const double beta[3],
+                           double M, double a, double out[49]);
+/* Blackbody(; wb_temperature, table_min, table_max, table_size) LUT
+ * (src/blackbody.jl:42-62): float[n][3]. */
+int st_blackbody_lut(double wb_temperature, double tmin, double tmax, int n, float* out);
+/* Seeded synthetic equirectangular sky: smooth gradient plus nstars Poisson
+ * stars with log-normal brightness (sigma), float[h][w][3].  Generator:
+ * SplitMix64-seeded xoshiro256**. */
+int st_make_sky(uint64_t seed, int w, int h, int nstars, double sigma, float* out);
+/* Seeded log-normal turbulent density grid float[nz][nphi][nr] (volumetric
+ * disc input). */
+int st_make_volume(uint64_t seed, int nr, int nphi, int nz, double sigma, float* out);
+
+const char* st_last_error(void);
+const char* st_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

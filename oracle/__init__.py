@@ -1,0 +1,272 @@
+"""Python handle on the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's
+cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+
+
+class OraScene(C.Structure):
+    _fields_ = [
+        ("M", C.c_double), ("a", C.c_double), ("kerr", C.c_int), ("r_escape", C.c_double),
+        ("relativistic", C.c_int),
+        ("dt", C.c_double), ("h_coef", C.c_double), ("h_floor", C.c_double), ("h_cap", C.c_double),
+        ("kerr_shrink", C.c_int), ("capture_mode", C.c_int), ("capture_factor", C.c_double),
+        ("rkill2", C.c_double), ("nmax", C.c_int), ("esc_interp", C.c_int), ("bardeen", C.c_int),
+        ("disc_on", C.c_int), ("disc_inner", C.c_double), ("disc_outer", C.c_double),
+        ("disc_falloff", C.c_double), ("lut_tmin", C.c_double), ("lut_tmax", C.c_double), ("lut_n", C.c_int),
+        ("lut", _D), ("T0", C.c_double), ("Tr0", C.c_double), ("T_hold_isco", C.c_int),
+        ("sky", _F), ("sky_w", C.c_int), ("sky_h", C.c_int),
+        ("star_params", C.c_double * 16), ("star_lut", _D), ("star_lut_n", C.c_int),
+        ("vol_on", C.c_int), ("vol", _F), ("nr", C.c_int), ("nphi", C.c_int), ("nz", C.c_int),
+        ("vol_params", C.c_double * 9),
+        ("cam", C.c_double * 49), ("width", C.c_int), ("height", C.c_int),
+        ("ju", C.c_double), ("jv", C.c_double), ("jw", C.c_double), ("jseed", C.c_int),
+    ]
+
+
+class OraRay(C.Structure):
+    _fields_ = [
+        ("fate", C.c_int), ("steps", C.c_int), ("rhs_evals", C.c_int),
+        ("dir", C.c_double * 3), ("rgb", C.c_double * 3), ("acc", C.c_double * 3), ("alpha", C.c_double),
+        ("ham", C.c_double), ("r_end", C.c_double), ("rk_min", C.c_double), ("ncross", C.c_int),
+        ("captured0", C.c_int), ("graze0", C.c_int), ("rc", C.c_double), ("Rc", C.c_double),
+        ("Rpp", C.c_double), ("p_t", C.c_double),
+    ]
+
+
+_SIG = {
+    "ora_rhs": (None, [C.c_int, _D, C.c_double, C.c_double, C.c_double, _D]),
+    "ora_ks_radius": (C.c_double, [C.c_double] * 4),
+    "ora_hamiltonian": (C.c_double, [C.c_int, _D, C.c_double, C.c_double, C.c_double]),
+    "ora_horizon_radius": (C.c_double, [C.c_double, C.c_double]),
+    "ora_photon_orbit_min": (C.c_double, [C.c_double, C.c_double]),
+    "ora_isco": (None, [C.c_double, C.c_double, _D]),
+    "ora_emitter_opz": (C.c_double, [C.c_double] * 10),
+    "ora_camera_basis": (None, [_D, _D, _D, _D, _D, _D]),
+    "ora_camera_tetrad": (None, [_D, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D, _D, _D]),
+    "ora_cam_params": (None, [_D, _D, _D, C.c_double, _D, C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_int, _D]),
+    "ora_init_ray": (None, [C.POINTER(OraScene), C.c_int, C.c_int, _D, _D]),
+    "ora_bardeen": (None, [C.POINTER(OraScene), _D, C.c_double, C.POINTER(OraRay)]),
+    "ora_blackbody_rgb": (None, [C.c_double, _D]),
+    "ora_blackbody_lut": (None, [C.c_double, C.c_double, C.c_double, C.c_int, _D]),
+    "ora_sample_background": (None, [_F, C.c_int, C.c_int, C.c_double, C.c_double, _D]),
+    "ora_sample_volume": (C.c_double, [C.POINTER(OraScene), C.c_double, C.c_double, C.c_double]),
+    "ora_sim_hash": (C.c_double, [C.c_int, C.c_int, C.c_int]),
+    "ora_starfield": (None, [C.POINTER(OraScene), C.c_double, C.c_double, C.c_double, _D]),
+    "ora_trace_rk4": (C.c_int, [C.POINTER(OraScene), C.c_int, C.c_int, C.POINTER(OraRay)]),
+    "ora_trace_tsit5": (C.c_int, [C.POINTER(OraScene), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(OraRay)]),
+    "ora_render": (C.c_longlong, [C.POINTER(OraScene), C.c_int, C.c_double, C.c_int, C.c_int, _F, C.c_double,
+                                  C.c_int]),
+    "ora_trace_list": (None, [C.POINTER(OraScene), C.c_int, C.c_double, C.c_int, _I, _D, C.c_int]),
+    "ora_tsit5_tableau": (None, [_D, _D, _D, _D]),
+}
+
+_lib = None
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def d(a) -> "C.Array":
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+# ---------------------------------------------------------------------------
+# primitives
+# ---------------------------------------------------------------------------
+def rhs(kerr: bool, s6, p_t: float, M: float, a: float) -> np.ndarray:
+    s, sp = d(s6)
+    out = np.zeros(6)
+    lib().ora_rhs(int(kerr), sp, p_t, M, a, out.ctypes.data_as(_D))
+    return out
+
+
+def hamiltonian(kerr: bool, s6, p_t, M, a) -> float:
+    s, sp = d(s6)
+    return lib().ora_hamiltonian(int(kerr), sp, p_t, M, a)
+
+
+def camera_tetrad(pos, fwd, right, up, M, a=0.0, beta=(0.0, 0.0, 0.0)):
+    arrs = [d(v) for v in (pos, fwd, right, up, beta)]
+    outs = [np.zeros(4) for _ in range(4)]
+    lib().ora_camera_tetrad(arrs[0][1], arrs[1][1], arrs[2][1], arrs[3][1], M, a, arrs[4][1],
+                            *[o.ctypes.data_as(_D) for o in outs])
+    return tuple(outs)  # u, Ef, Er, Eu
+
+
+def cam_params(pos, target, up, fov, velocity=(0.0, 0.0, 0.0), M=1.0, a=0.0, fisheye_deg=0.0, focus_dist=1.0,
+               equirect=False) -> np.ndarray:
+    arrs = [d(v) for v in (pos, target, up, velocity)]
+    out = np.zeros(49)
+    lib().ora_cam_params(arrs[0][1], arrs[1][1], arrs[2][1], fov, arrs[3][1], M, a, fisheye_deg, focus_dist,
+                         int(equirect), out.ctypes.data_as(_D))
+    return out
+
+
+def isco(M, a):
+    out = np.zeros(3)
+    lib().ora_isco(M, a, out.ctypes.data_as(_D))
+    return out
+
+
+def blackbody_rgb(T):
+    out = np.zeros(3)
+    lib().ora_blackbody_rgb(T, out.ctypes.data_as(_D))
+    return out
+
+
+def blackbody_lut(wb=6500.0, tmin=500.0, tmax=30000.0, n=1024) -> np.ndarray:
+    out = np.zeros((n, 3))
+    lib().ora_blackbody_lut(wb, tmin, tmax, n, out.ctypes.data_as(_D))
+    return out
+
+
+def sample_background(sky: np.ndarray, theta, phi) -> np.ndarray:
+    sky = np.ascontiguousarray(sky, dtype=np.float32)
+    out = np.zeros(3)
+    lib().ora_sample_background(sky.ctypes.data_as(_F), sky.shape[1], sky.shape[0], theta, phi,
+                                out.ctypes.data_as(_D))
+    return out
+
+
+def sim_hash(x, y, z) -> float:
+    return lib().ora_sim_hash(int(x), int(y), int(z))
+
+
+def tsit5_tableau():
+    c, a, b, bt = np.zeros(7), np.zeros(49), np.zeros(7), np.zeros(7)
+    lib().ora_tsit5_tableau(*(v.ctypes.data_as(_D) for v in (c, a, b, bt)))
+    return c, a.reshape(7, 7), b, bt
+
+
+# ---------------------------------------------------------------------------
+# scenes
+# ---------------------------------------------------------------------------
+class Scene:
+    """An ora_scene plus the arrays it points into (kept alive here)."""
+
+    def __init__(self, *, M, a, cam_pos, cam_target, cam_up, fov, width, height, velocity=(0.0, 0.0, 0.0),
+                 r_escape=100.0, relativistic=False, dt=1.0, h_coef=0.05, h_floor=0.0, h_cap=1e30,
+                 kerr_shrink=False, capture_mode=1, capture_factor=1.01, nmax=6000, esc_interp=True,
+                 bardeen=True, disc=None, sky=None, stars=None, volume=None, cam=None, ju=0.5, jv=0.5,
+                 jw=0.0, jseed=0, fisheye_deg=0.0, focus_dist=1.0, equirect=False):
+        s = OraScene()
+        s.M, s.a = float(M), float(a)
+        s.kerr = int(a != 0.0)
+        s.r_escape = float(r_escape)
+        s.relativistic = int(relativistic)
+        s.dt, s.h_coef, s.h_floor, s.h_cap = float(dt), float(h_coef), float(h_floor), float(h_cap)
+        s.kerr_shrink = int(kerr_shrink)
+        s.capture_mode, s.capture_factor = int(capture_mode), float(capture_factor)
+        rk = 0.995 * lib().ora_photon_orbit_min(M, a)
+        s.rkill2 = float(np.float32(np.float32(rk) * np.float32(rk)))
+        s.nmax, s.esc_interp, s.bardeen = int(nmax), int(esc_interp), int(bardeen)
+        self._keep = []
+        if disc is not None:
+            lut = np.ascontiguousarray(disc["lut"], dtype=np.float64)
+            self._keep.append(lut)
+            s.disc_on = 1
+            s.disc_inner, s.disc_outer, s.disc_falloff = disc["inner"], disc["outer"], disc.get("falloff", 0.0)
+            s.lut_tmin, s.lut_tmax, s.lut_n = disc.get("tmin", 500.0), disc.get("tmax", 30000.0), lut.shape[0]
+            s.lut = lut.ctypes.data_as(_D)
+            s.T0, s.Tr0, s.T_hold_isco = disc["T0"], disc["r0"], int(disc["hold"])
+        else:
+            s.T0, s.Tr0, s.T_hold_isco = 1e4, 3.0, 0
+        if sky is not None:
+            sk = np.ascontiguousarray(sky, dtype=np.float32)
+            self._keep.append(sk)
+            s.sky, s.sky_w, s.sky_h = sk.ctypes.data_as(_F), sk.shape[1], sk.shape[0]
+        if stars is not None:
+            sp, slut = stars
+            slut = np.ascontiguousarray(slut, dtype=np.float64)
+            self._keep.append(slut)
+            for k in range(16):
+                s.star_params[k] = float(sp[k])
+            s.star_lut, s.star_lut_n = slut.ctypes.data_as(_D), slut.shape[0]
+        if volume is not None:
+            dens, vp = volume
+            dens = np.ascontiguousarray(dens, dtype=np.float32)
+            self._keep.append(dens)
+            s.vol_on = 1
+            s.vol = dens.ctypes.data_as(_F)
+            s.nz, s.nphi, s.nr = dens.shape
+            for k in range(9):
+                s.vol_params[k] = float(vp[k])
+        if cam is None:
+            cam = cam_params(cam_pos, cam_target, cam_up, fov, velocity, M, a, fisheye_deg, focus_dist, equirect)
+        for k in range(49):
+            s.cam[k] = float(cam[k])
+        s.width, s.height = int(width), int(height)
+        s.ju, s.jv, s.jw, s.jseed = float(ju), float(jv), float(jw), int(jseed)
+        self.s = s
+
+    @property
+    def ptr(self):
+        return C.byref(self.s)
+
+    def init_ray(self, i, j):
+        s6 = np.zeros(6)
+        pt = C.c_double()
+        lib().ora_init_ray(self.ptr, i, j, s6.ctypes.data_as(_D), C.byref(pt))
+        return s6, pt.value
+
+    def trace(self, i, j, method="rk4", tol=1e-10) -> OraRay:
+        o = OraRay()
+        if method == "rk4":
+            lib().ora_trace_rk4(self.ptr, i, j, C.byref(o))
+        else:
+            lib().ora_trace_tsit5(self.ptr, i, j, tol, tol, 2000000, C.byref(o))
+        return o
+
+    def trace_list(self, ij: np.ndarray, method="rk4", tol=1e-10, nthreads=0) -> np.ndarray:
+        ij = np.ascontiguousarray(ij, dtype=np.int32)
+        rec = np.zeros((ij.shape[0], 16))
+        lib().ora_trace_list(self.ptr, 0 if method == "rk4" else 1, tol, ij.shape[0], ij.ctypes.data_as(_I),
+                             rec.ctypes.data_as(_D), nthreads)
+        return rec
+
+    def render(self, method="rk4", tol=1e-6, row0=0, rows=None, nthreads=0, weight=1.0):
+        rows = self.s.height - row0 if rows is None else rows
+        out = np.zeros((self.s.height, self.s.width, 3), dtype=np.float32)
+        steps = lib().ora_render(self.ptr, 0 if method == "rk4" else 1, tol, row0, rows, out.ctypes.data_as(_F),
+                                 weight, nthreads)
+        return out, steps
+
+
+REC_FIELDS = ["fate", "steps", "dx", "dy", "dz", "r", "g", "b", "ham", "r_end", "rk_min", "ncross",
+              "captured0", "graze0", "rc", "Rc"]

@@ -1,0 +1,1137 @@
+/*
+ * FP64 CPU oracle for the Kerr–Schild null-geodesic tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY — the checker, never the product.  Only tests/,
+ * the smoke check and bench.py's cpu_baseline leg load liboracle.so.
+ *
+ * Every function restates the reference algorithm in plain double precision
+ * and cites the reference file:line it follows.  See oracle.h for the list.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+inline double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+inline double sq(double x) { return x * x; }
+
+/* Julia `mod(x, y)` for y > 0: result in [0, y). */
+inline double jmod(double x, double y) {
+    double m = std::fmod(x, y);
+    if (m < 0) m += y;
+    if (m >= y) m -= y;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* ---------------------------------------------------------------------------
+ * Geodesic right-hand sides.
+ * Schwarzschild: src/metal.jl:324-340 (== src/preview.jl:44-55, src/gr.jl:85-109).
+ * Kerr:          src/metal.jl:466-508 (KS chart, implicit-r derivatives).
+ * ------------------------------------------------------------------------- */
+void ora_rhs(int kerr, const double s[6], double p_t, double M, double a, double o[6]) {
+    const double x = s[0], y = s[1], z = s[2], px = s[3], py = s[4], pz = s[5];
+    if (!kerr) {
+        const double r2 = x * x + y * y + z * z;
+        const double inv_r = 1.0 / std::sqrt(r2);
+        const double f = 2.0 * M * inv_r;
+        const double kap = (x * px + y * py + z * pz) * inv_r;
+        const double ell = -p_t + kap;
+        const double c1 = f * ell * inv_r;
+        const double c2 = f * ell * (0.5 * ell + kap) * inv_r * inv_r;
+        o[0] = px - c1 * x;
+        o[1] = py - c1 * y;
+        o[2] = pz - c1 * z;
+        o[3] = c1 * px - c2 * x;
+        o[4] = c1 * py - c2 * y;
+        o[5] = c1 * pz - c2 * z;
+        return;
+    }
+    const double a2 = a * a, z2 = z * z;
+    const double w = x * x + y * y + z2 - a2;
+    double r2 = 0.5 * (w + std::sqrt(w * w + 4.0 * a2 * z2));
+    r2 = std::max(r2, 1e-12);
+    const double r = std::sqrt(r2);
+    const double r3 = r2 * r;
+    const double invS = 1.0 / (r2 * r2 + a2 * z2);
+    const double R2A = r2 + a2;
+    const double iRA = 1.0 / R2A;
+    const double inv_r = 1.0 / r;
+    const double Lx = (r * x + a * y) * iRA;
+    const double Ly = (r * y - a * x) * iRA;
+    const double Lz = z * inv_r;
+    const double ell = -p_t + Lx * px + Ly * py + Lz * pz;
+    const double f = 2.0 * M * r3 * invS;
+    const double fl = f * ell;
+    const double drx = x * r3 * invS;
+    const double dry = y * r3 * invS;
+    const double drz = z * r * R2A * invS;
+    const double dfdr = 2.0 * M * r2 * (3.0 * a2 * z2 - r2 * r2) * invS * invS;
+    const double dfz0 = -4.0 * M * r3 * a2 * z * invS * invS;
+    const double dlx0 = (r * px - a * py) * iRA;
+    const double dly0 = (a * px + r * py) * iRA;
+    const double dlz0 = pz * inv_r;
+    const double dldr = (px * (x * R2A - 2.0 * r * (r * x + a * y)) +
+                         py * (y * R2A - 2.0 * r * (r * y - a * x))) * iRA * iRA -
+                        z * pz * inv_r * inv_r;
+    const double hl2 = 0.5 * ell * ell;
+    o[0] = px - fl * Lx;
+    o[1] = py - fl * Ly;
+    o[2] = pz - fl * Lz;
+    o[3] = hl2 * (dfdr * drx) + fl * (dldr * drx + dlx0);
+    o[4] = hl2 * (dfdr * dry) + fl * (dldr * dry + dly0);
+    o[5] = hl2 * (dfdr * drz + dfz0) + fl * (dldr * drz + dlz0);
+}
+
+/* KS radius from the implicit quartic r^4 - (rho^2 - a^2) r^2 - a^2 z^2 = 0
+ * (src/gr.jl:139-141, src/metal.jl:468-471). */
+double ora_ks_radius(double x, double y, double z, double a) {
+    const double a2 = a * a;
+    const double w = x * x + y * y + z * z - a2;
+    const double r2 = 0.5 * (w + std::sqrt(w * w + 4.0 * a2 * z * z));
+    return std::sqrt(std::max(r2, 0.0));
+}
+
+/* H = 1/2 p g^-1 p with g^-1 = eta - f l(x)l  (src/gr.jl:119-153, 196-203). */
+double ora_hamiltonian(int kerr, const double s[6], double p_t, double M, double a) {
+    const double x = s[0], y = s[1], z = s[2], px = s[3], py = s[4], pz = s[5];
+    double f, Lx, Ly, Lz;
+    if (!kerr) {
+        const double r = std::sqrt(x * x + y * y + z * z);
+        f = 2.0 * M / r;
+        Lx = x / r; Ly = y / r; Lz = z / r;
+    } else {
+        const double a2 = a * a;
+        const double w = x * x + y * y + z * z - a2;
+        const double r2 = 0.5 * (w + std::sqrt(w * w + 4.0 * a2 * z * z));
+        const double r = std::sqrt(std::max(r2, 1e-300));
+        f = 2.0 * M * r2 * r / (r2 * r2 + a2 * z * z);
+        const double R2A = r2 + a2;
+        Lx = (r * x + a * y) / R2A; Ly = (r * y - a * x) / R2A; Lz = z / r;
+    }
+    const double ell = -p_t + Lx * px + Ly * py + Lz * pz;
+    return 0.5 * (-p_t * p_t + px * px + py * py + pz * pz - f * ell * ell);
+}
+
+/* src/gr.jl:42-43 */
+double ora_horizon_radius(double M, double a) { return M + std::sqrt(std::max(M * M - a * a, 0.0)); }
+
+/* src/gr.jl:58-60 */
+double ora_photon_orbit_min(double M, double a) {
+    if (a == 0.0) return 3.0 * M;
+    return 2.0 * M * (1.0 + std::cos((2.0 / 3.0) * std::acos(-a / M)));
+}
+
+/* Prograde ISCO and plunge constants, src/metal.jl:371-380 (BPT 1972). */
+void ora_isco(double M, double a, double out[3]) {
+    auto cb = [](double v) { return std::cbrt(std::max(v, 1e-12)); };
+    const double q = a / M;
+    const double z1 = 1.0 + cb(1.0 - q * q) * (cb(1.0 + q) + cb(1.0 - q));
+    const double z2 = std::sqrt(3.0 * q * q + z1 * z1);
+    const double r_isco = M * (3.0 + z2 - std::sqrt(std::max((3.0 - z1) * (3.0 + z1 + 2.0 * z2), 0.0)));
+    out[0] = r_isco;
+    out[1] = std::sqrt(1.0 - 2.0 * M / (3.0 * r_isco));
+    out[2] = (2.0 * M / (3.0 * std::sqrt(3.0))) * (1.0 + 2.0 * std::sqrt(3.0 * r_isco / M - 2.0));
+}
+
+/* Emitter shift 1+z = -p.u/E, circular orbit outside the ISCO, plunge inside,
+ * src/metal.jl:396-428. */
+double ora_emitter_opz(double sx, double sy, double psx, double psy, double E,
+                       double M, double a, double r_isco, double E_isco, double L_isco) {
+    const double a2 = a * a;
+    const double s2 = sx * sx + sy * sy;
+    const double r = std::sqrt(std::max(s2 - a2, 1e-6));
+    const double sqM = std::sqrt(M), sqr = std::sqrt(r), r32 = r * sqr;
+    const double Delta = r * r - 2.0 * M * r + a2;
+    double ut, uphi, ur = 0.0;
+    if (r >= r_isco) {
+        const double den = r32 - 3.0 * M * sqr + 2.0 * a * sqM;
+        ut = (r32 + a * sqM) / (std::sqrt(r32) * std::sqrt(std::max(den, 1e-2)));
+        uphi = sqM / (r32 + a * sqM) * ut;
+    } else {
+        const double inv_D = 1.0 / std::max(Delta, 1e-3);
+        ut = ((r * r + a2 + 2.0 * M * a2 / r) * E_isco - 2.0 * M * a * L_isco / r) * inv_D;
+        uphi = (2.0 * M * a * E_isco / r + (1.0 - 2.0 * M / r) * L_isco) * inv_D;
+        const double Rr = sq(E_isco * (r * r + a2) - a * L_isco) -
+                          Delta * (r * r + sq(L_isco - a * E_isco));
+        ur = -std::sqrt(std::max(Rr, 0.0)) / (r * r);
+        ut += 2.0 * M * r * inv_D * ur;
+        uphi += a * inv_D * ur;
+    }
+    const double inv_s2 = 1.0 / std::max(s2, 1e-6);
+    const double cphi = (r * sx + a * sy) * inv_s2;
+    const double sphi = (r * sy - a * sx) * inv_s2;
+    const double ux = ur * cphi - sy * uphi;
+    const double uy = ur * sphi + sx * uphi;
+    const double Es = std::fabs(E) > 1e-6 ? E : 1e-6;
+    return clampd(ut - (psx * ux + psy * uy) / Es, 0.1, 20.0);
+}
+
+/* Camera(pos, target, up): src/camera.jl:44-49. */
+void ora_camera_basis(const double pos[3], const double target[3], const double up[3],
+                      double fwd[3], double right[3], double upl[3]) {
+    double f[3] = {target[0] - pos[0], target[1] - pos[1], target[2] - pos[2]};
+    double n = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (int k = 0; k < 3; ++k) fwd[k] = f[k] / n;
+    double r[3] = {fwd[1] * up[2] - fwd[2] * up[1], fwd[2] * up[0] - fwd[0] * up[2],
+                   fwd[0] * up[1] - fwd[1] * up[0]};
+    n = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int k = 0; k < 3; ++k) right[k] = r[k] / n;
+    upl[0] = right[1] * fwd[2] - right[2] * fwd[1];
+    upl[1] = right[2] * fwd[0] - right[0] * fwd[2];
+    upl[2] = right[0] * fwd[1] - right[1] * fwd[0];
+}
+
+/* Orthonormal camera tetrad in KS coordinates, src/preview.jl:93-198:
+ * static observer (r >= 2.5M) / radial faller inside (a = 0); static<->ZAMO
+ * smoothstep blend on f in [0.72, 0.88] (a != 0); Gram–Schmidt under g with
+ * forward first; optional Lorentz boost, |beta| <= 0.99. */
+void ora_camera_tetrad(const double pos[3], const double fwd[3], const double right[3],
+                       const double up[3], double M, double a, const double beta[3],
+                       double u[4], double Ef[4], double Er[4], double Eu[4]) {
+    double l[3], f;
+    if (a == 0.0) {
+        const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+        const double xh[3] = {pos[0] / r, pos[1] / r, pos[2] / r};
+        f = 2.0 * M / r;
+        const double E_freeze2 = 1.0 - 2.0 / 2.5;
+        const double E = std::sqrt(std::max(E_freeze2, 1.0 - f));
+        const double v = -std::sqrt(std::max(E * E - (1.0 - f), 0.0));
+        const double w = (1.0 - E * (E - v)) / (E - v);
+        const double lu = E + w;
+        u[0] = E + f * lu;
+        for (int k = 0; k < 3; ++k) u[k + 1] = (w - f * lu) * xh[k];
+        for (int k = 0; k < 3; ++k) l[k] = xh[k];
+    } else {
+        const double x = pos[0], y = pos[1], z = pos[2];
+        const double a2 = a * a;
+        const double wq = x * x + y * y + z * z - a2;
+        const double rk2 = 0.5 * (wq + std::sqrt(wq * wq + 4.0 * a2 * z * z));
+        const double rk = std::sqrt(std::max(rk2, 1e-12));
+        const double iRA = 1.0 / (rk2 + a2);
+        l[0] = (rk * x + a * y) * iRA;
+        l[1] = (rk * y - a * x) * iRA;
+        l[2] = z / rk;
+        const double Sq = rk2 * rk2 + a2 * z * z;
+        f = 2.0 * M * rk2 * rk / Sq;
+        const double Delta = std::max(rk2 - 2.0 * M * rk + a2, 1e-6);
+        const double kk = 2.0 * M * rk / Delta;
+        const double gr[3] = {x * rk2 * rk / Sq, y * rk2 * rk / Sq, z * rk * (rk2 + a2) / Sq};
+        const double luc = 1.0 + kk * (l[0] * gr[0] + l[1] * gr[1] + l[2] * gr[2]);
+        double uz[4];
+        uz[0] = 1.0 + f * luc;
+        for (int k = 0; k < 3; ++k) uz[k + 1] = kk * gr[k] - (f * luc) * l[k];
+        const double lu_z = uz[0] + l[0] * uz[1] + l[1] * uz[2] + l[2] * uz[3];
+        const double nrm2 = -uz[0] * uz[0] + uz[1] * uz[1] + uz[2] * uz[2] + uz[3] * uz[3] + f * lu_z * lu_z;
+        double uzamo[4];
+        for (int k = 0; k < 4; ++k) uzamo[k] = uz[k] / std::sqrt(-nrm2);
+        if (f >= 0.88) {
+            for (int k = 0; k < 4; ++k) u[k] = uzamo[k];
+        } else {
+            const double ustat[4] = {1.0 / std::sqrt(1.0 - f), 0.0, 0.0, 0.0};
+            double wb = clampd((f - 0.72) / 0.16, 0.0, 1.0);
+            wb = wb * wb * (3.0 - 2.0 * wb);
+            if (wb == 0.0) {
+                for (int k = 0; k < 4; ++k) u[k] = ustat[k];
+            } else {
+                double um[4];
+                for (int k = 0; k < 4; ++k) um[k] = (1.0 - wb) * ustat[k] + wb * uzamo[k];
+                const double lu = um[0] + l[0] * um[1] + l[1] * um[2] + l[2] * um[3];
+                const double nn = -um[0] * um[0] + um[1] * um[1] + um[2] * um[2] + um[3] * um[3] + f * lu * lu;
+                for (int k = 0; k < 4; ++k) u[k] = um[k] / std::sqrt(-nn);
+            }
+        }
+    }
+    auto ldot = [&](const double A[4]) { return A[0] + l[0] * A[1] + l[1] * A[2] + l[2] * A[3]; };
+    auto gdot = [&](const double A[4], const double B[4]) {
+        return -A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + A[3] * B[3] + f * ldot(A) * ldot(B);
+    };
+    double ef[4] = {0.0, fwd[0], fwd[1], fwd[2]};
+    double er[4] = {0.0, right[0], right[1], right[2]};
+    double eu[4] = {0.0, up[0], up[1], up[2]};
+    double c = gdot(ef, u);
+    for (int k = 0; k < 4; ++k) ef[k] += c * u[k];
+    c = std::sqrt(gdot(ef, ef));
+    for (int k = 0; k < 4; ++k) ef[k] /= c;
+    c = gdot(er, u);
+    for (int k = 0; k < 4; ++k) er[k] += c * u[k];
+    c = gdot(er, ef);
+    for (int k = 0; k < 4; ++k) er[k] -= c * ef[k];
+    c = std::sqrt(gdot(er, er));
+    for (int k = 0; k < 4; ++k) er[k] /= c;
+    c = gdot(eu, u);
+    for (int k = 0; k < 4; ++k) eu[k] += c * u[k];
+    const double c1 = gdot(eu, ef), c2 = gdot(eu, er);
+    for (int k = 0; k < 4; ++k) eu[k] = eu[k] - c1 * ef[k] - c2 * er[k];
+    c = std::sqrt(gdot(eu, eu));
+    for (int k = 0; k < 4; ++k) eu[k] /= c;
+
+    double b2 = beta[0] * beta[0] + beta[1] * beta[1] + beta[2] * beta[2];
+    if (b2 > 1e-12) {
+        const double b2c = std::min(b2, 0.9801);
+        const double sc = std::sqrt(b2c / b2);
+        const double bt[3] = {beta[0] * sc, beta[1] * sc, beta[2] * sc};
+        const double g = 1.0 / std::sqrt(1.0 - b2c);
+        double bE[4], ub[4];
+        for (int k = 0; k < 4; ++k) bE[k] = bt[0] * ef[k] + bt[1] * er[k] + bt[2] * eu[k];
+        for (int k = 0; k < 4; ++k) ub[k] = g * (u[k] + bE[k]);
+        const double kk = (g - 1.0) / b2c;
+        for (int k = 0; k < 4; ++k) {
+            const double t = kk * bE[k] + g * u[k];
+            Ef[k] = ef[k] + bt[0] * t;
+            Er[k] = er[k] + bt[1] * t;
+            Eu[k] = eu[k] + bt[2] * t;
+        }
+        for (int k = 0; k < 4; ++k) u[k] = ub[k];
+        return;
+    }
+    for (int k = 0; k < 4; ++k) { Ef[k] = ef[k]; Er[k] = er[k]; Eu[k] = eu[k]; }
+}
+
+/* cam_params[49] packing, src/metal.jl:1739-1774; camera_beta src/camera.jl:19-22. */
+void ora_cam_params(const double pos[3], const double target[3], const double up[3],
+                    double fov, const double velocity[3], double M, double a,
+                    double fisheye_deg, double focus_dist, int equirect, double out[49]) {
+    double fwd[3], right[3], upl[3];
+    ora_camera_basis(pos, target, up, fwd, right, upl);
+    double beta[3] = {0, 0, 0};
+    if (velocity) {
+        beta[0] = velocity[0] * fwd[0] + velocity[1] * fwd[1] + velocity[2] * fwd[2];
+        beta[1] = velocity[0] * right[0] + velocity[1] * right[1] + velocity[2] * right[2];
+        beta[2] = velocity[0] * upl[0] + velocity[1] * upl[1] + velocity[2] * upl[2];
+    }
+    double u[4], Ef[4], Er[4], Eu[4];
+    ora_camera_tetrad(pos, fwd, right, upl, M, a, beta, u, Ef, Er, Eu);
+    for (int k = 0; k < 49; ++k) out[k] = 0.0;
+    out[0] = pos[0]; out[1] = pos[1]; out[2] = pos[2];
+    out[3] = fov;
+    for (int k = 0; k < 4; ++k) {
+        out[4 + k] = Ef[k]; out[8 + k] = Er[k]; out[12 + k] = Eu[k]; out[16 + k] = u[k];
+    }
+    out[20] = equirect ? 2.0 : (fisheye_deg > 0 ? 1.0 : 0.0);
+    out[21] = std::max(fisheye_deg, 0.0) * M_PI / 180.0;
+    out[24] = focus_dist;
+}
+
+/* _sim_hash, src/disc_sim.jl:83-91: integer hash -> [0,1) in 2^-24 steps. */
+double ora_sim_hash(int x, int y, int z) {
+    uint32_t h = (uint32_t)x * 0x8da6b343u ^ (uint32_t)y * 0xd8163841u ^ (uint32_t)z * 0xcb1ab31fu;
+    h ^= h >> 13;
+    h *= 0x9e3779b9u;
+    h ^= h >> 16;
+    return (double)(h & 0x00ffffffu) * 5.9604644775390625e-8;
+}
+
+/* Ray generation + KS photon init, src/metal.jl:833-995 (pixel (i,j) 0-based;
+ * the reference's hashes take the 1-based indices). */
+void ora_init_ray(const ora_scene* sc, int i, int j, double s[6], double* p_t_out) {
+    double cp[49];
+    std::memcpy(cp, sc->cam, sizeof(cp));
+    const int I = i + 1, J = j + 1;
+    if (cp[28] > 0.0) {  /* per-pixel shutter, src/metal.jl:833-848 */
+        const double tau = ora_sim_hash(I, J, (int)cp[28]);
+        for (int k = 0; k < 20; ++k) {
+            if (k == 3) continue;  /* fov is not interpolated */
+            cp[k] += tau * (sc->cam[29 + k] - cp[k]);
+        }
+    }
+    const double fw = sc->width, fh = sc->height;
+    double ju = sc->ju, jv = sc->jv;
+    if (sc->jw > 0.0) {  /* per-pixel decorrelated jitter, src/metal.jl:857-862 */
+        ju += ora_sim_hash(I, J, sc->jseed) * sc->jw;
+        jv += ora_sim_hash(I, J, sc->jseed + 271) * sc->jw;
+    }
+    const double M = sc->M;
+    const double half_h = fh / 2.0;
+    const double u = ((double)I - 1.0 + ju - fw / 2.0) / half_h;
+    const double v = ((double)J - 1.0 + jv - fh / 2.0) / half_h;
+    double offr = 0.0, offu = 0.0;
+    if (cp[25] > 0.0) {  /* stratified thin lens, src/metal.jl:881-903 */
+        const int sd = (int)cp[27];
+        const double u01 = cp[22] + ora_sim_hash(I, J, sd) * cp[25];
+        const double v01 = cp[23] + ora_sim_hash(I, J, sd + 7919) * cp[25];
+        const double ox = 2.0 * u01 - 1.0, oy = 2.0 * v01 - 1.0;
+        if (ox != 0.0 || oy != 0.0) {
+            double rr, th;
+            if (std::fabs(ox) > std::fabs(oy)) { rr = ox; th = 0.785398 * (oy / ox); }
+            else { rr = oy; th = 1.570796 - 0.785398 * (ox / oy); }
+            offr = cp[26] * rr * std::cos(th);
+            offu = cp[26] * rr * std::sin(th);
+        }
+    }
+    double cr, cu, cf;
+    if (cp[20] > 1.5) {  /* equirect bake projection, src/metal.jl:910-926 */
+        const double phe = u * 1.5707963, the = (v + 1.0) * 1.5707963;
+        const double st = std::sin(the);
+        cf = std::cos(the); cr = st * std::cos(phe); cu = st * std::sin(phe);
+    } else if (cp[20] > 0.5) {  /* equidistant fisheye, src/metal.jl:927-934 */
+        const double rho = std::sqrt(u * u + v * v);
+        const double tp = rho * cp[21];
+        const double st = std::sin(tp);
+        const double inv_rho = rho > 1e-8 ? 1.0 / rho : 0.0;
+        cr = st * u * inv_rho; cu = st * v * inv_rho; cf = std::cos(tp);
+    } else {  /* rectilinear pinhole (+ thin-lens refocus), src/metal.jl:935-949 */
+        double dx = u * cp[3], dy = v * cp[3];
+        double nu = std::sqrt(dx * dx + dy * dy + 1.0);
+        if (offr != 0.0 || offu != 0.0) {
+            const double k = nu / cp[24];
+            dx -= offr * k; dy -= offu * k;
+            nu = std::sqrt(dx * dx + dy * dy + 1.0);
+        }
+        cr = dx / nu; cu = dy / nu; cf = 1.0 / nu;
+    }
+    const double x = cp[0] + offr * cp[9] + offu * cp[13];
+    const double y = cp[1] + offr * cp[10] + offu * cp[14];
+    const double z = cp[2] + offr * cp[11] + offu * cp[15];
+    const double qt = cf * cp[4] + cr * cp[8] + cu * cp[12] - cp[16];
+    const double qx = cf * cp[5] + cr * cp[9] + cu * cp[13] - cp[17];
+    const double qy = cf * cp[6] + cr * cp[10] + cu * cp[14] - cp[18];
+    const double qz = cf * cp[7] + cr * cp[11] + cu * cp[15] - cp[19];
+    double p_t, px, py, pz;
+    if (sc->kerr) {
+        const double a = sc->a, a2 = a * a;
+        const double r2e = x * x + y * y + z * z;
+        const double wl = r2e - a2;
+        const double rk2 = 0.5 * (wl + std::sqrt(wl * wl + 4.0 * a2 * z * z));
+        const double rk = std::sqrt(std::max(rk2, 1e-12));
+        const double iRA = 1.0 / (rk2 + a2);
+        const double lx = (rk * x + a * y) * iRA, ly = (rk * y - a * x) * iRA, lz = z / rk;
+        const double fl = 2.0 * M * rk2 * rk / (rk2 * rk2 + a2 * z * z);
+        const double lq = qt + lx * qx + ly * qy + lz * qz;
+        const double flq = fl * lq;
+        p_t = -qt + flq;
+        px = qx + flq * lx; py = qy + flq * ly; pz = qz + flq * lz;
+    } else {
+        const double r = std::sqrt(x * x + y * y + z * z);
+        const double f = 2.0 * M / r;
+        const double lq = qt + (x * qx + y * qy + z * qz) / r;
+        p_t = -qt + f * lq;
+        const double flr = f * lq / r;
+        px = qx + flr * x; py = qy + flr * y; pz = qz + flr * z;
+    }
+    s[0] = x; s[1] = y; s[2] = z; s[3] = px; s[4] = py; s[5] = pz;
+    *p_t_out = p_t;
+}
+
+/* Bardeen launch-time capture test, src/metal.jl:1008-1094, evaluated for
+ * both a != 0 and a = 0 (the reference consults it only for Kerr).  Also
+ * returns the largest critical point rc of the radial potential in
+ * (r_plus, r_cam) — the photon-shell radius this ray grazes — with R(rc)
+ * and R''(rc). */
+void ora_bardeen(const ora_scene* sc, const double s[6], double p_t, ora_ray* o) {
+    o->captured0 = 0; o->graze0 = 0; o->rc = -1.0; o->Rc = 0.0; o->Rpp = 0.0;
+    const double M = sc->M, a = sc->kerr ? sc->a : 0.0;
+    const double x = s[0], y = s[1], z = s[2], px = s[3], py = s[4], pz = s[5];
+    const double E0 = -p_t;
+    if (std::fabs(E0) <= 1e-12) return;
+    const double a2 = a * a;
+    const double aw = x * x + y * y + z * z - a2;
+    const double rk2 = 0.5 * (aw + std::sqrt(aw * aw + 4.0 * a2 * z * z));
+    const double rk = std::sqrt(std::max(rk2, 1e-12));
+    const double cth = z / rk;
+    const double s2 = std::max(1.0 - cth * cth, 1e-8);
+    const double sth = std::sqrt(s2);
+    const double Lz0 = x * py - y * px;
+    const double pth = (cth / sth) * (x * px + y * py) - rk * sth * pz;
+    const double Q0 = pth * pth + cth * cth * (Lz0 * Lz0 / s2 - a2 * E0 * E0);
+    const double lam = Lz0 / E0, eta = Q0 / (E0 * E0);
+    const double c2 = a2 - lam * lam - eta;
+    const double c1 = 2.0 * M * (eta + (lam - a) * (lam - a));
+    const double c0 = -a2 * eta;
+    const double rplus = ora_horizon_radius(M, a);
+    double v[6];
+    ora_rhs(a != 0.0, s, p_t, M, a, v);
+    const double rdot = rk2 * (x * v[0] + y * v[1]) + (rk2 + a2) * z * v[2];
+    if (!(rdot < 0.0)) return;
+    auto R = [&](double r) { return ((r * r + c2) * r + c1) * r + c0; };
+    const double pb = 0.5 * c2, qb = 0.25 * c1;
+    const double Db = 0.25 * qb * qb + pb * pb * pb / 27.0;
+    bool cap = true, graze = false;
+    double best = -1.0;
+    if (Db >= 0.0) {
+        const double sD = std::sqrt(Db);
+        const double rc = std::cbrt(-0.5 * qb + sD) + std::cbrt(-0.5 * qb - sD);
+        if (rc > rplus && rc < rk) {
+            const double Rv = R(rc);
+            cap = Rv > 0.0;
+            if (std::fabs(Rv) < 0.08 * std::max(rc * rc * rc * rc, 1.0)) graze = true;
+            best = rc;
+        }
+    } else {
+        const double mb = 2.0 * std::sqrt(-pb / 3.0);
+        const double ac = clampd(3.0 * qb / (pb * mb), -1.0, 1.0);
+        const double thb = std::acos(ac) / 3.0;
+        for (int k = 0; k < 3; ++k) {
+            const double rc = mb * std::cos(thb - 2.0943951023931953 * k);
+            if (rc > rplus && rc < rk) {
+                const double Rv = R(rc);
+                if (Rv <= 0.0) cap = false;
+                if (std::fabs(Rv) < 0.08 * std::max(rc * rc * rc * rc, 1.0)) graze = true;
+                best = std::max(best, rc);
+            }
+        }
+    }
+    o->captured0 = cap ? 1 : 0;
+    o->graze0 = graze ? 1 : 0;
+    o->rc = best;
+    if (best > 0) { o->Rc = R(best); o->Rpp = 12.0 * best * best + 2.0 * c2; }
+}
+
+/* blackbody_rgb, src/blackbody.jl:15-35 (CIE Gaussian fits, XYZ -> linear sRGB,
+ * max-normalised). */
+static double cie_gauss(double lam, double mu, double s1, double s2) {
+    const double s = lam < mu ? s1 : s2;
+    return std::exp(-0.5 * sq((lam - mu) / s));
+}
+void ora_blackbody_rgb(double T, double out[3]) {
+    double X = 0, Y = 0, Z = 0;
+    for (int k = 0; k <= 80; ++k) {
+        const double lam = 380.0 + 5.0 * k;
+        const double lm = lam * 1e-9;
+        const double B = 1.0 / (std::pow(lm, 5) * (std::exp(0.014388 / (lm * T)) - 1.0));
+        const double xb = 1.056 * cie_gauss(lam, 599.8, 37.9, 31.0) + 0.362 * cie_gauss(lam, 442.0, 16.0, 26.7) -
+                          0.065 * cie_gauss(lam, 501.1, 20.4, 26.2);
+        const double yb = 0.821 * cie_gauss(lam, 568.8, 46.9, 40.5) + 0.286 * cie_gauss(lam, 530.9, 16.3, 31.1);
+        const double zb = 1.217 * cie_gauss(lam, 437.0, 11.8, 36.0) + 0.681 * cie_gauss(lam, 459.0, 26.0, 13.8);
+        X += B * xb; Y += B * yb; Z += B * zb;
+    }
+    const double r = std::max(3.2406 * X - 1.5372 * Y - 0.4986 * Z, 0.0);
+    const double g = std::max(-0.9689 * X + 1.8758 * Y + 0.0415 * Z, 0.0);
+    const double b = std::max(0.0557 * X - 0.2040 * Y + 1.0570 * Z, 0.0);
+    const double m = std::max(std::max(r, g), std::max(b, 1e-10));
+    out[0] = r / m; out[1] = g / m; out[2] = b / m;
+}
+
+/* Blackbody LUT, src/blackbody.jl:42-62: white-balanced, `range(tmin, tmax,
+ * length=n)` temperatures. */
+void ora_blackbody_lut(double wb, double tmin, double tmax, int n, double* out) {
+    double ref[3];
+    ora_blackbody_rgb(wb, ref);
+    for (int k = 0; k < n; ++k) {
+        const double T = n > 1 ? tmin + (tmax - tmin) * (double)k / (double)(n - 1) : tmin;
+        double c[3];
+        ora_blackbody_rgb(std::max(T, 1.0), c);
+        for (int ch = 0; ch < 3; ++ch) out[3 * k + ch] = c[ch] * (1.0 / ref[ch]);
+    }
+}
+
+/* Bilinear sky fetch, src/metal.jl:523-561 (no phi wrap at the seam). */
+void ora_sample_background(const float* bg, int W, int H, double theta, double phi, double out[3]) {
+    const double v = clampd(theta / M_PI, 0.0, 1.0);
+    const double u = clampd(jmod(phi, 2.0 * M_PI) / (2.0 * M_PI), 0.0, 1.0);
+    const double xf = u * (W - 1), yf = v * (H - 1);
+    int x0 = (int)std::floor(xf); x0 = std::min(std::max(x0, 0), W - 1);
+    const int x1 = std::min(x0 + 1, W - 1);
+    int y0 = (int)std::floor(yf); y0 = std::min(std::max(y0, 0), H - 1);
+    const int y1 = std::min(y0 + 1, H - 1);
+    const double fx = xf - x0, fy = yf - y0;
+    const double w00 = (1 - fx) * (1 - fy), w10 = fx * (1 - fy), w01 = (1 - fx) * fy, w11 = fx * fy;
+    for (int c = 0; c < 3; ++c) {
+        out[c] = bg[(y0 * W + x0) * 3 + c] * w00 + bg[(y0 * W + x1) * 3 + c] * w10 +
+                 bg[(y1 * W + x0) * 3 + c] * w01 + bg[(y1 * W + x1) * 3 + c] * w11;
+    }
+}
+
+/* Trilinear density, src/metal.jl:569-610 (== src/disc_volume.jl:486-509). */
+double ora_sample_volume(const ora_scene* sc, double s, double phi, double z) {
+    const double* vp = sc->vol_params;
+    const double zmax = vp[2];
+    if (s <= 0.0 || std::fabs(z) >= zmax) return 0.0;
+    const double ls = std::log(s);
+    if (ls <= vp[0] || ls >= vp[1]) return 0.0;
+    const int nr = sc->nr, nphi = sc->nphi, nz = sc->nz;
+    const double fr = (ls - vp[0]) / (vp[1] - vp[0]) * (nr - 1);
+    const double fp = jmod(phi, 2.0 * M_PI) / (2.0 * M_PI) * nphi;
+    const double fz = (z + zmax) / (2.0 * zmax) * (nz - 1);
+    const int i0 = std::min(std::max((int)std::floor(fr), 0), nr - 2);
+    const double tr = fr - i0;
+    const int j0 = (int)std::floor(fp);
+    const double tp = fp - j0;
+    const int k0 = std::min(std::max((int)std::floor(fz), 0), nz - 2);
+    const double tz = fz - k0;
+    const int j0a = ((j0 % nphi) + nphi) % nphi;
+    const int j1a = (((j0 + 1) % nphi) + nphi) % nphi;
+    auto d = [&](int ii, int jj, int kk) { return (double)sc->vol[((size_t)kk * nphi + jj) * nr + ii]; };
+    const double c00 = d(i0, j0a, k0) + tr * (d(i0 + 1, j0a, k0) - d(i0, j0a, k0));
+    const double c10 = d(i0, j1a, k0) + tr * (d(i0 + 1, j1a, k0) - d(i0, j1a, k0));
+    const double c01 = d(i0, j0a, k0 + 1) + tr * (d(i0 + 1, j0a, k0 + 1) - d(i0, j0a, k0 + 1));
+    const double c11 = d(i0, j1a, k0 + 1) + tr * (d(i0 + 1, j1a, k0 + 1) - d(i0, j1a, k0 + 1));
+    const double c0 = c00 + tp * (c10 - c00), c1 = c01 + tp * (c11 - c01);
+    return c0 + tz * (c1 - c0);
+}
+
+/* Procedural point stars, src/metal.jl:652-724. */
+void ora_starfield(const ora_scene* sc, double dx, double dy, double dz, double out[3]) {
+    const double* sp = sc->star_params;
+    const double N = sp[1], fill = sp[2], sig = sp[3], flux0 = sp[4];
+    const double gnx = sp[5], gny = sp[6], gnz = sp[7], gconc = sp[8];
+    const double tmin = sp[9], tspan = sp[10], lut_tmin = sp[11], lut_tmax = sp[12], lut_size = sp[13];
+    const int seed = (int)sp[14];
+    const double two_pi = 2.0 * M_PI;
+    const double u = std::atan2(dy, dx) / two_pi + 0.5;
+    const double v = (dz + 1.0) * 0.5;
+    const int Ni = (int)N;
+    const int i0 = (int)std::floor(u * N), j0 = (int)std::floor(v * N);
+    const double inv2s2 = 1.0 / (2.0 * sig * sig), cut = 25.0 * sig * sig;
+    out[0] = out[1] = out[2] = 0.0;
+    for (int dj = -1; dj <= 1; ++dj) {
+        const int jj = j0 + dj;
+        if (jj < 0 || jj >= Ni) continue;
+        for (int di = -1; di <= 1; ++di) {
+            const int ii = (((i0 + di) % Ni) + Ni) % Ni;
+            const double su = ora_sim_hash(ii, jj, seed + 1), sv = ora_sim_hash(ii, jj, seed + 2);
+            const double phs = (((double)ii + su) / N - 0.5) * two_pi;
+            const double sz = 2.0 * ((double)jj + sv) / N - 1.0;
+            const double sr = std::sqrt(std::max(1.0 - sz * sz, 0.0));
+            const double sx = sr * std::cos(phs), sy = sr * std::sin(phs);
+            const double ex = dx - sx, ey = dy - sy, ez = dz - sz;
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            if (d2 > cut) continue;
+            double occ = fill;
+            if (gconc > 0.0) occ *= std::exp(-std::fabs(sx * gnx + sy * gny + sz * gnz) * gconc);
+            if (!(ora_sim_hash(ii, jj, seed) < occ)) continue;
+            const double xi = std::max(ora_sim_hash(ii, jj, seed + 3), 1e-4);
+            const double flux = flux0 * std::exp(-0.6666667 * std::log(xi));
+            const double w = flux * std::exp(-d2 * inv2s2);
+            const double T = tmin + tspan * ora_sim_hash(ii, jj, seed + 4);
+            const double frac = (clampd(T, lut_tmin, lut_tmax) - lut_tmin) / std::max(lut_tmax - lut_tmin, 1e-6);
+            int li = (int)(frac * (lut_size - 1.0) + 0.5);
+            li = std::min(std::max(li, 0), sc->star_lut_n - 1);
+            for (int c = 0; c < 3; ++c) out[c] += w * sc->star_lut[3 * li + c];
+        }
+    }
+}
+
+}  // extern "C"
+
+/* ---------------------------------------------------------------------------
+ * Shared per-ray shading state.
+ * ------------------------------------------------------------------------- */
+namespace {
+
+struct Shade {
+    double acc[3] = {0, 0, 0};
+    double alpha = 1.0;
+    int ncross = 0;
+    double march_acc = 0.0;
+};
+
+inline double temp_law(const ora_scene* sc, double s, double r_isco) {
+    const double sT = sc->T_hold_isco ? std::max(s, r_isco) : s;
+    return sc->T0 * std::exp(-0.75 * std::log(std::max(sT / sc->Tr0, 1e-6)));
+}
+
+inline void lut_color(const ora_scene* sc, double T_obs, double col[3]) {
+    const double frac = (clampd(T_obs, sc->lut_tmin, sc->lut_tmax) - sc->lut_tmin) / (sc->lut_tmax - sc->lut_tmin);
+    int li = (int)(frac * (sc->lut_n - 1.0) + 0.5);
+    li = std::min(std::max(li, 0), sc->lut_n - 1);
+    for (int c = 0; c < 3; ++c) col[c] = sc->lut[3 * li + c];
+}
+
+/* Thin-plane crossing shading at (xh, yh) with covariant momentum
+ * (pxh, pyh): src/metal.jl:1524-1569. */
+void shade_disc(const ora_scene* sc, double xh, double yh, double pxh, double pyh, double p_t,
+                double scam, const double isco[3], Shade& sh) {
+    const double M = sc->M;
+    const double s = std::sqrt(xh * xh + yh * yh);
+    if (!(sc->disc_inner < s && s < sc->disc_outer)) return;
+    const double R = s / (2.0 * M);
+    const double T_emit = temp_law(sc, s, isco[0]);
+    const double plunge = (sc->T_hold_isco && s < isco[0]) ? std::pow(s / isco[0], 3) : 1.0;
+    const double a_eff = sc->kerr ? sc->a : 0.0;
+    const double opz = ora_emitter_opz(xh, yh, pxh, pyh, -p_t, M, a_eff, isco[0], isco[1], isco[2]);
+    const double T_obs = T_emit * scam / opz;
+    const double inten = 100.0 / (std::exp(29622.4 / std::max(T_obs, 1.0)) - 1.0);
+    double col[3];
+    lut_color(sc, T_obs, col);
+    const double R_in = sc->disc_inner / (2.0 * M), R_out = sc->disc_outer / (2.0 * M);
+    const double iscotaper = clampd((R * R - R_in * R_in) * 0.3, 0.0, 1.0);
+    const double outertaper = clampd(T_emit / 1000.0, 0.0, 1.0);
+    const double density = clampd((R_out - R) / (R_out - R_in), 0.0, 1.0);
+    const double dpow = density <= 1e-6 ? (sc->disc_falloff > 0.0 ? 0.0 : 1.0)
+                                        : std::exp(sc->disc_falloff * std::log(density));
+    const double opacity = iscotaper * outertaper * dpow * plunge;
+    const double w = sh.alpha * opacity * inten;
+    for (int c = 0; c < 3; ++c) sh.acc[c] += w * col[c];
+    sh.alpha *= (1.0 - opacity);
+    sh.ncross++;
+}
+
+/* One gas sample at (sx, sy, sz) with momentum (psx, psy): src/metal.jl:1457-1505. */
+void shade_gas(const ora_scene* sc, double sx, double sy, double sz, double psx, double psy,
+               double p_t, double scam, const double isco[3], double ds_gas, Shade& sh) {
+    if (!(std::fabs(sz) < sc->vol_params[2])) return;
+    const double s_cyl = std::sqrt(sx * sx + sy * sy);
+    if (!(s_cyl > 1e-6)) return;
+    double rho = ora_sample_volume(sc, s_cyl, std::atan2(sy, sx), sz);
+    if (!(rho > 1e-4)) return;
+    const double M = sc->M;
+    const double T_emit = temp_law(sc, s_cyl, isco[0]);
+    if (s_cyl < isco[0]) rho *= std::pow(s_cyl / isco[0], 3);
+    const double a_eff = sc->kerr ? sc->a : 0.0;
+    const double opz = ora_emitter_opz(sx, sy, psx, psy, -p_t, M, a_eff, isco[0], isco[1], isco[2]);
+    const double T_obs = T_emit * scam / opz;
+    const double inten = 100.0 / (std::exp(29622.4 / std::max(T_obs, 1.0)) - 1.0);
+    double col[3];
+    lut_color(sc, T_obs, col);
+    const double tau = sc->vol_params[7] * rho * ds_gas;
+    const double aa = 1.0 - std::exp(-tau);
+    const double w = sh.alpha * aa * inten * sc->vol_params[6];
+    for (int c = 0; c < 3; ++c) sh.acc[c] += w * col[c];
+    sh.alpha *= (1.0 - aa);
+}
+
+/* Arc-length gas march over one step from (xp..) to (x..), src/metal.jl:1440-1513. */
+void march_gas(const ora_scene* sc, const double sp[6], const double sn[6], double seg, double p_t,
+               double scam, const double isco[3], double ds_gas, Shade& sh) {
+    if (!(seg >= 0.0 && seg < 1e6)) return;
+    double t_s = ds_gas - sh.march_acc;
+    if (t_s > seg) { sh.march_acc += seg; return; }
+    while (t_s <= seg && sh.alpha > 0.003) {
+        const double fs = t_s / seg;
+        const double sx = sp[0] + fs * (sn[0] - sp[0]);
+        const double sy = sp[1] + fs * (sn[1] - sp[1]);
+        const double sz = sp[2] + fs * (sn[2] - sp[2]);
+        const double psx = sp[3] + fs * (sn[3] - sp[3]);
+        const double psy = sp[4] + fs * (sn[4] - sp[4]);
+        shade_gas(sc, sx, sy, sz, psx, psy, p_t, scam, isco, ds_gas, sh);
+        t_s += ds_gas;
+    }
+    sh.march_acc = seg - (t_s - ds_gas);
+}
+
+/* Fate + sky, src/metal.jl:1629-1704. */
+void finish(const ora_scene* sc, const double s[6], double p_t, double scam, bool hit, bool nonfinite,
+            bool captured0, const double* vstart, const double* xstart, Shade& sh, ora_ray* o) {
+    const double M = sc->M;
+    const double rf = std::max(std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]), 1e-6);
+    o->r_end = rf;
+    o->alpha = sh.alpha;
+    for (int c = 0; c < 3; ++c) o->acc[c] = sh.acc[c];
+    o->ncross = sh.ncross;
+    o->ham = nonfinite ? NAN : ora_hamiltonian(sc->kerr, s, p_t, M, sc->a);
+    if (hit || captured0 || rf < 4.0 * M) {
+        o->fate = nonfinite ? 2 : 1;
+        for (int c = 0; c < 3; ++c) { o->rgb[c] = sh.acc[c]; o->dir[c] = 0.0; }
+        return;
+    }
+    double v[6];
+    ora_rhs(sc->kerr, s, p_t, M, sc->a, v);
+    double vl = std::max(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), 1e-20);
+    double d[3] = {v[0] / vl, v[1] / vl, v[2] / vl};
+    if (vstart && xstart) {
+        /* direction interpolated to |x| = r_escape across the last step */
+        const double r0 = std::sqrt(xstart[0] * xstart[0] + xstart[1] * xstart[1] + xstart[2] * xstart[2]);
+        if (r0 < sc->r_escape && rf > sc->r_escape) {
+            const double t = (sc->r_escape - r0) / (rf - r0);
+            double v0l = std::max(std::sqrt(vstart[0] * vstart[0] + vstart[1] * vstart[1] + vstart[2] * vstart[2]), 1e-20);
+            double dd[3];
+            for (int c = 0; c < 3; ++c) dd[c] = vstart[c] / v0l + t * (d[c] - vstart[c] / v0l);
+            const double n = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+            for (int c = 0; c < 3; ++c) d[c] = dd[c] / n;
+        }
+    }
+    for (int c = 0; c < 3; ++c) o->dir[c] = d[c];
+    const double th = std::acos(clampd(d[2], -1.0, 1.0));
+    const double ph = std::atan2(d[1], d[0]);
+    double col[3] = {0, 0, 0};
+    if (sc->sky) ora_sample_background(sc->sky, sc->sky_w, sc->sky_h, th, ph, col);
+    if (sc->star_params[0] > 0.0 && sc->star_lut) {
+        const double tw = sc->star_params[15];
+        double st[3];
+        ora_starfield(sc, d[0], d[1], d[2], st);
+        for (int c = 0; c < 3; ++c) col[c] = col[c] * tw + sc->star_params[0] * st[c];
+    }
+    if (scam != 1.0) {
+        col[0] *= 57.4 / (std::exp(4.067 / scam) - 1.0);
+        col[1] *= 90.2 / (std::exp(4.513 / scam) - 1.0);
+        col[2] *= 206.5 / (std::exp(5.335 / scam) - 1.0);
+    }
+    o->fate = 0;
+    for (int c = 0; c < 3; ++c) o->rgb[c] = sh.acc[c] + sh.alpha * col[c];
+}
+
+struct RayStart {
+    double s[6];
+    double p_t, scam, isco[3], rplus, rcap, r_floor;
+    bool captured0;
+};
+
+void ray_start(const ora_scene* sc, int i, int j, ora_ray* o, RayStart& rs) {
+    std::memset(o, 0, sizeof(*o));
+    ora_init_ray(sc, i, j, rs.s, &rs.p_t);
+    o->p_t = rs.p_t;
+    rs.scam = sc->relativistic ? 1.0 / clampd(std::fabs(rs.p_t), 0.05, 20.0) : 1.0;
+    ora_isco(sc->M, sc->kerr ? sc->a : 0.0, rs.isco);
+    ora_bardeen(sc, rs.s, rs.p_t, o);
+    rs.captured0 = sc->kerr && sc->bardeen && sc->a != 0.0 && o->captured0;
+    rs.rplus = ora_horizon_radius(sc->M, sc->kerr ? sc->a : 0.0);
+    rs.rcap = sc->capture_factor * rs.rplus;
+    const double r0 = std::sqrt(rs.s[0] * rs.s[0] + rs.s[1] * rs.s[1] + rs.s[2] * rs.s[2]);
+    rs.r_floor = r0 > 2.05 * sc->M ? 2.0 * sc->M : 0.3 * sc->M;
+}
+
+inline double ks_r2(const ora_scene* sc, const double s[6]) {
+    if (!sc->kerr) return s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double a2 = sc->a * sc->a;
+    const double aw = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] - a2;
+    return 0.5 * (aw + std::sqrt(aw * aw + 4.0 * a2 * s[2] * s[2]));
+}
+
+}  // namespace
+
+extern "C" {
+
+/* Fixed-step RK4, the kernel's algorithm in FP64: src/metal.jl:1201-1260
+ * (kill/escape/step rule), 1382-1401 (RK4), 1414-1421 (NaN bail),
+ * 1440-1571 (gas, thin disc), 1629-1704 (fate, sky). */
+int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
+    RayStart rs;
+    ray_start(sc, i, j, o, rs);
+    const double M = sc->M;
+    double s[6];
+    std::memcpy(s, rs.s, sizeof(s));
+    Shade sh;
+    bool hit = false, nonfinite = false, escaped_loop = false;
+    double r_prev = -1.0;
+    double xstart[3], vstart[3];
+    bool have_prev = false;
+    o->rk_min = 1e30;
+    const bool vol = sc->vol_on != 0;
+    const bool disc_plane = sc->disc_on && sc->disc_inner > 0.0 && sc->disc_outer > sc->disc_inner && !vol;
+    const double vol_zmax = sc->vol_params[2];
+    const double vol_s_out = std::exp(sc->vol_params[1]);
+    const double vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
+    const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
+    const double ds_gas = mstep * sc->dt;
+    int steps = 0;
+    for (int it = 0; it < sc->nmax; ++it) {
+        const double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+        const double r = std::sqrt(r2);
+        const double rk2 = ks_r2(sc, s);
+        o->rk_min = std::min(o->rk_min, std::sqrt(std::max(rk2, 0.0)));
+        if (sc->capture_mode == 1) {
+            if (rk2 < rs.rcap * rs.rcap) { hit = true; break; }
+        } else if (sc->kerr) {
+            if (rk2 < sc->rkill2) { hit = true; break; }
+        } else if (r < 3.2 * M) {
+            if (r < rs.r_floor || (r < 2.95 * M && r_prev > 0.0 && r < r_prev - 1e-4 * M)) { hit = true; break; }
+            r_prev = r;
+        }
+        if (r > sc->r_escape) { escaped_loop = true; break; }
+        const double hcap = (vol && r2 < vol_rb2) ? 2.0 : sc->h_cap;
+        double h = sc->dt * std::min(std::max(sc->h_coef * r / M, sc->h_floor), hcap);
+        if (sc->kerr && sc->kerr_shrink) {
+            const double fr = clampd((rk2 - sc->rkill2) / std::max(sc->rkill2, 1e-6), 0.0, 1.0);
+            h *= 0.12 + 0.88 * fr;
+        }
+        double sp[6];
+        std::memcpy(sp, s, sizeof(sp));
+        double k1[6], k2[6], k3[6], k4[6], t[6];
+        /* Capture mode 1 also captures a ray whose RK stage point falls
+         * inside the capture radius: below the prograde photon orbit no
+         * turning point exists, and a backward ray nearing r_plus has a
+         * covariant momentum that diverges at finite affine parameter, so a
+         * step that straddles the radius must not be completed. */
+        double stage_rk2 = 1e300;
+        ora_rhs(sc->kerr, s, rs.p_t, M, sc->a, k1);
+        for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k1[c];
+        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k2);
+        for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k2[c];
+        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k3);
+        for (int c = 0; c < 6; ++c) t[c] = s[c] + h * k3[c];
+        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k4);
+        o->rhs_evals += 4;
+        if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap) { hit = true; break; }
+        for (int c = 0; c < 6; ++c) s[c] += (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
+        ++steps;
+        for (int c = 0; c < 3; ++c) { xstart[c] = sp[c]; vstart[c] = k1[c]; }
+        have_prev = true;
+        if (!(s[0] == s[0]) || !(s[2] == s[2]) || !(s[3] == s[3]) ||
+            !(std::fabs(s[0]) + std::fabs(s[1]) + std::fabs(s[2]) < 1e30)) { hit = true; nonfinite = true; break; }
+        const double vlen = std::max(std::sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]), 1e-20);
+        if (vol && sh.alpha > 0.003 && std::min(r2, s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) < vol_rb2) {
+            march_gas(sc, sp, s, h * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
+            if (sh.alpha < 0.003) break;
+        }
+        if (disc_plane && sh.alpha > 0.001 && sp[2] * s[2] < 0.0) {
+            const double cf = sp[2] / (sp[2] - s[2]);
+            const double xh = sp[0] + cf * (s[0] - sp[0]);
+            const double yh = sp[1] + cf * (s[1] - sp[1]);
+            const double pxh = sp[3] + cf * (s[3] - sp[3]);
+            const double pyh = sp[4] + cf * (s[4] - sp[4]);
+            shade_disc(sc, xh, yh, pxh, pyh, rs.p_t, rs.scam, rs.isco, sh);
+        }
+    }
+    o->steps = steps;
+    const bool interp = sc->esc_interp && escaped_loop && have_prev;
+    finish(sc, s, rs.p_t, rs.scam, hit, nonfinite, rs.captured0, interp ? vstart : nullptr,
+           interp ? xstart : nullptr, sh, o);
+    return o->fate;
+}
+
+}  // extern "C"
+
+/* ---------------------------------------------------------------------------
+ * Adaptive Tsitouras 5(4) with exact event location.
+ * Coefficients: Ch. Tsitouras, "Runge–Kutta pairs of order 5(4) satisfying
+ * only the first column simplifying assumption", Comput. Math. Appl. 62
+ * (2011) 770–775 — the same tableau as src/metal.jl:1317-1358 (there in
+ * Float32).
+ * ------------------------------------------------------------------------- */
+namespace {
+
+const double TC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+const double TA[7][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774, 0}};
+const double TB[7] = {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742,
+                      -3.290069515436081, 2.324710524099774, 0.0};
+const double TBT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                       -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                       0.015151515151515152};
+
+struct Tsit {
+    const ora_scene* sc;
+    double p_t;
+    long long evals = 0;
+    void rhs(const double s[6], double o[6]) {
+        ora_rhs(sc->kerr, s, p_t, sc->M, sc->a, o);
+        ++evals;
+    }
+    /* One step of size h from s (k1 = RHS at s given).  Returns the 5th-order
+     * solution in sn and the scaled error norm. */
+    double step(const double s[6], const double k1[6], double h, double sn[6], double rtol, double atol) {
+        double k[7][6];
+        std::memcpy(k[0], k1, sizeof(double) * 6);
+        double t[6];
+        for (int st = 1; st < 7; ++st) {
+            for (int c = 0; c < 6; ++c) {
+                double acc = 0.0;
+                for (int m = 0; m < st; ++m) acc += TA[st][m] * k[m][c];
+                t[c] = s[c] + h * acc;
+            }
+            rhs(t, k[st]);
+        }
+        double err = 0.0;
+        for (int c = 0; c < 6; ++c) {
+            double acc = 0.0, e = 0.0;
+            for (int m = 0; m < 7; ++m) { acc += TB[m] * k[m][c]; e += TBT[m] * k[m][c]; }
+            sn[c] = s[c] + h * acc;
+            const double scl = atol + rtol * std::max(std::fabs(s[c]), std::fabs(sn[c]));
+            err = std::max(err, std::fabs(h * e) / scl);
+        }
+        return err;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void ora_tsit5_tableau(double c[7], double a[49], double b[7], double btilde[7]) {
+    for (int i = 0; i < 7; ++i) {
+        c[i] = TC[i]; b[i] = TB[i]; btilde[i] = TBT[i];
+        for (int j = 0; j < 7; ++j) a[i * 7 + j] = TA[i][j];
+    }
+}
+
+/* The same physics and termination rules as ora_trace_rk4, but integrated
+ * adaptively with every event (escape radius, capture radius, z = 0 disc
+ * crossing) located exactly by re-stepping from the step start. */
+int ora_trace_tsit5(const ora_scene* sc, int i, int j, double rtol, double atol, int max_steps, ora_ray* o) {
+    RayStart rs;
+    ray_start(sc, i, j, o, rs);
+    const double M = sc->M;
+    Tsit ts{sc, rs.p_t};
+    double s[6];
+    std::memcpy(s, rs.s, sizeof(s));
+    Shade sh;
+    const bool vol = sc->vol_on != 0;
+    const bool disc_plane = sc->disc_on && sc->disc_inner > 0.0 && sc->disc_outer > sc->disc_inner && !vol;
+    const double vol_zmax = sc->vol_params[2];
+    const double vol_s_out = std::exp(sc->vol_params[1]);
+    const double vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
+    const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
+    const double ds_gas = mstep * sc->dt;
+    bool hit = false, nonfinite = false;
+    o->rk_min = std::sqrt(std::max(ks_r2(sc, s), 0.0));
+    const double r0 = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    if (r0 > sc->r_escape) {  /* camera outside the escape sphere */
+        finish(sc, s, rs.p_t, rs.scam, false, false, rs.captured0, nullptr, nullptr, sh, o);
+        return o->fate;
+    }
+    /* capture event function: positive outside */
+    auto gcap = [&](const double* st) -> double {
+        if (sc->capture_mode == 1) return std::sqrt(std::max(ks_r2(sc, st), 0.0)) - rs.rcap;
+        if (sc->kerr) return ks_r2(sc, st) - sc->rkill2;
+        return std::sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]) - 2.95 * M;
+    };
+    auto gesc = [&](const double* st) -> double {
+        return std::sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]) - sc->r_escape;
+    };
+    double k1[6];
+    ts.rhs(s, k1);
+    double r = r0;
+    double hh = 0.01 * std::max(r, 1.0);
+    int steps = 0;
+    bool done = false;
+    while (!done && steps < max_steps) {
+        r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        const double hmax = 0.1 * std::max(r, 0.5);
+        hh = std::min(hh, hmax);
+        double sn[6];
+        const double err = ts.step(s, k1, hh, sn, rtol, atol);
+        if (!(err == err) || !(sn[0] == sn[0]) || !(sn[3] == sn[3])) {
+            if (hh < 1e-12) { hit = true; nonfinite = true; break; }
+            hh *= 0.25;
+            continue;
+        }
+        if (err > 1.0) {
+            hh *= std::max(0.2, 0.9 * std::pow(err, -0.2));
+            continue;
+        }
+        /* accepted [s -> sn] with step hh; look for events in it */
+        const double gc0 = gcap(s), gc1 = gcap(sn);
+        const double ge0 = gesc(s), ge1 = gesc(sn);
+        const bool ev_cap = gc0 > 0.0 && gc1 <= 0.0 &&
+                            (sc->capture_mode == 1 || sc->kerr ||
+                             (std::sqrt(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]) <
+                              std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2])));
+        const bool ev_esc = ge0 < 0.0 && ge1 >= 0.0;
+        const bool ev_disc = disc_plane && sh.alpha > 0.001 && s[2] * sn[2] < 0.0;
+        double t_ev = 2.0;
+        int which = -1;
+        auto locate = [&](auto g, double g0) -> double {
+            /* Illinois false position on theta in (0, 1] */
+            double ta = 0.0, tb = 1.0, fa = g0, fb;
+            double tmp[6];
+            ts.step(s, k1, hh, tmp, rtol, atol);
+            fb = g(tmp);
+            int side = 0;
+            double tc = 1.0;
+            for (int itn = 0; itn < 80; ++itn) {
+                tc = (ta * fb - tb * fa) / (fb - fa);
+                if (!(tc > ta && tc < tb)) tc = 0.5 * (ta + tb);
+                ts.step(s, k1, tc * hh, tmp, rtol, atol);
+                const double fc = g(tmp);
+                if (fc == 0.0 || (tb - ta) < 1e-15) break;
+                if ((fc > 0) == (fb > 0)) {
+                    tb = tc; fb = fc;
+                    if (side == -1) fa *= 0.5;
+                    side = -1;
+                } else {
+                    ta = tc; fa = fc;
+                    if (side == 1) fb *= 0.5;
+                    side = 1;
+                }
+            }
+            return tc;
+        };
+        if (ev_cap) {
+            const double t = locate(gcap, gc0);
+            if (t < t_ev) { t_ev = t; which = 0; }
+        }
+        if (ev_esc) {
+            const double t = locate(gesc, ge0);
+            if (t < t_ev) { t_ev = t; which = 1; }
+        }
+        if (ev_disc) {
+            auto gz = [&](const double* st) { return st[2]; };
+            const double t = locate(gz, s[2]);
+            if (t < t_ev) { t_ev = t; which = 2; }
+        }
+        double snew[6];
+        double hused = hh;
+        if (which >= 0) {
+            hused = t_ev * hh;
+            ts.step(s, k1, hused, snew, rtol, atol);
+        } else {
+            std::memcpy(snew, sn, sizeof(snew));
+        }
+        ++steps;
+        if (vol && sh.alpha > 0.003 &&
+            std::min(s[0] * s[0] + s[1] * s[1] + s[2] * s[2],
+                     snew[0] * snew[0] + snew[1] * snew[1] + snew[2] * snew[2]) < vol_rb2) {
+            const double vlen = std::max(std::sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]), 1e-20);
+            march_gas(sc, s, snew, hused * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
+            if (sh.alpha < 0.003) { std::memcpy(s, snew, sizeof(s)); break; }
+        }
+        std::memcpy(s, snew, sizeof(s));
+        o->rk_min = std::min(o->rk_min, std::sqrt(std::max(ks_r2(sc, s), 0.0)));
+        if (which == 0) { hit = true; break; }
+        if (which == 1) { done = true; break; }
+        if (which == 2) {
+            s[2] = 0.0;  /* exactly on the plane */
+            shade_disc(sc, s[0], s[1], s[3], s[4], rs.p_t, rs.scam, rs.isco, sh);
+        }
+        ts.rhs(s, k1);
+        if (which < 0) {
+            hh = hh * std::min(4.0, std::max(0.2, 0.9 * std::pow(std::max(err, 1e-10), -0.2)));
+        }
+    }
+    o->steps = steps;
+    o->rhs_evals = (int)std::min<long long>(ts.evals, 2000000000LL);
+    finish(sc, s, rs.p_t, rs.scam, hit, nonfinite, rs.captured0, nullptr, nullptr, sh, o);
+    return o->fate;
+}
+
+long long ora_render(const ora_scene* sc, int method, double tol, int row0, int rows, float* out,
+                     double weight, int nthreads) {
+    const int W = sc->width;
+    long long total = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+#endif
+    for (int jj = row0; jj < row0 + rows; ++jj) {
+        for (int ii = 0; ii < W; ++ii) {
+            ora_ray o;
+            if (method == 1) ora_trace_tsit5(sc, ii, jj, tol, tol, 1000000, &o);
+            else ora_trace_rk4(sc, ii, jj, &o);
+            total += o.steps;
+            float* px = out + ((size_t)jj * W + ii) * 3;
+            for (int c = 0; c < 3; ++c) px[c] += (float)(weight * o.rgb[c]);
+        }
+    }
+    return total;
+}
+
+void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const int* ij, double* rec,
+                    int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int k = 0; k < n; ++k) {
+        ora_ray o;
+        if (method == 1) ora_trace_tsit5(sc, ij[2 * k], ij[2 * k + 1], tol, tol, 1000000, &o);
+        else ora_trace_rk4(sc, ij[2 * k], ij[2 * k + 1], &o);
+        double* r = rec + (size_t)k * 16;
+        r[0] = o.fate; r[1] = o.steps;
+        r[2] = o.dir[0]; r[3] = o.dir[1]; r[4] = o.dir[2];
+        r[5] = o.rgb[0]; r[6] = o.rgb[1]; r[7] = o.rgb[2];
+        r[8] = o.ham; r[9] = o.r_end; r[10] = o.rk_min; r[11] = o.ncross;
+        r[12] = o.captured0; r[13] = o.graze0; r[14] = o.rc; r[15] = o.Rc;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+/*
+ * FP64 CPU oracle for the Kerr–Schild null-geodesic tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; it is built into oracle/liboracle.so and used by tests/,
+ * the smoke check and bench.py's cpu_baseline leg as the checker.
+ *
+ * It restates, in plain double precision C++, the algorithm of SpaceTime.jl's
+ * per-pixel tracer (file:line citations are against the reference tree):
+ *   - ks_rhs_mtl / kerr_rhs_mtl          src/metal.jl:324-340, 466-508
+ *   - ks_camera_tetrad (+ Kerr blend, boost) src/preview.jl:93-198
+ *   - camera construction, Lens            src/camera.jl:36-50, src/raytrace.jl:22-29
+ *   - cam_params packing                   src/metal.jl:1739-1774
+ *   - ray generation + KS photon init      src/metal.jl:855-995
+ *   - Bardeen launch-time capture test     src/metal.jl:1008-1094
+ *   - isco_mtl, emitter_opz_mtl            src/metal.jl:371-428
+ *   - step rule, kill/escape, RK4 loop     src/metal.jl:1201-1260, 1382-1401
+ *   - thin disc crossing + Planck shading  src/metal.jl:1515-1571
+ *   - volumetric march + trilinear sample  src/metal.jl:1440-1513, 569-610
+ *   - fate + sky fetch                     src/metal.jl:1629-1704, 523-561
+ *   - procedural starfield, _sim_hash      src/metal.jl:652-724, src/disc_sim.jl:83-91
+ *   - blackbody_rgb + LUT                  src/blackbody.jl:15-80
+ *   - hamiltonian / metric_inverse         src/gr.jl:119-203
+ * and adds an adaptive Tsitouras 5(4) integrator (coefficients from
+ * Tsitouras 2011, the tableau the kernel's ORD 45 path carries at
+ * src/metal.jl:1278-1381) with exact event location, used as the
+ * high-accuracy truth for the GPU parity tests.
+ */
+#ifndef SPACETIME_ORACLE_H
+#define SPACETIME_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* spacetime */
+    double M, a;
+    int kerr;              /* 1: Kerr RHS (a != 0); 0: Schwarzschild RHS */
+    double r_escape;       /* Euclidean |x| escape radius */
+    int relativistic;      /* camera/infinity shift 1/|p_t| on shading */
+    /* step rule: h = dt * min(max(coef*r/M, floor), cap) */
+    double dt, h_coef, h_floor, h_cap;
+    int kerr_shrink;       /* h *= 0.12 + 0.88 fr near the reference kill radius */
+    int capture_mode;      /* 0: reference kill rules; 1: KS r < capture_factor * r_plus */
+    double capture_factor;
+    double rkill2;         /* reference Kerr kill radius^2 (KS r) */
+    int nmax;
+    int esc_interp;        /* escape direction interpolated to |x| = r_escape */
+    int bardeen;           /* Kerr launch-time capture verdict forces shadow */
+    /* thin disc */
+    int disc_on;
+    double disc_inner, disc_outer, disc_falloff;
+    double lut_tmin, lut_tmax;
+    int lut_n;
+    const double* lut;     /* [lut_n][3] */
+    double T0, Tr0;        /* T_emit = T0 * (max(s, hold ? r_isco : 0) / Tr0)^-3/4 */
+    int T_hold_isco;
+    /* sky: [sky_h][sky_w][3] float32, row = theta, column = phi */
+    const float* sky;
+    int sky_w, sky_h;
+    /* stars */
+    double star_params[16];
+    const double* star_lut;
+    int star_lut_n;
+    /* volume: [nz][nphi][nr] float32 (Julia (nr,nphi,nz) column-major) */
+    int vol_on;
+    const float* vol;
+    int nr, nphi, nz;
+    double vol_params[9];
+    /* camera */
+    double cam[49];
+    int width, height;
+    double ju, jv;         /* sub-pixel sample point (pass stratum origin when jw > 0) */
+    double jw;             /* per-pixel hashed jitter width (0 = off) */
+    int jseed;             /* per-pixel jitter hash seed */
+} ora_scene;
+
+typedef struct {
+    int fate;              /* 0 escaped, 1 captured, 2 non-finite */
+    int steps;             /* accepted integrator steps */
+    int rhs_evals;
+    double dir[3];         /* unit escape direction (escaped rays) */
+    double rgb[3];         /* sample radiance (acc + alpha*sky, or acc) */
+    double acc[3];
+    double alpha;
+    double ham;            /* Hamiltonian at termination */
+    double r_end;          /* Euclidean |x| at termination */
+    double rk_min;         /* minimum KS r along the path */
+    int ncross;            /* shaded disc crossings */
+    int captured0, graze0; /* Bardeen launch verdicts */
+    double rc;             /* Bardeen critical radius (photon-shell radius of this ray), -1 if none */
+    double Rc;             /* R(rc)/E^2 */
+    double Rpp;            /* R''(rc)/E^2 */
+    double p_t;
+} ora_ray;
+
+/* physics primitives */
+void ora_rhs(int kerr, const double s[6], double p_t, double M, double a, double out[6]);
+double ora_ks_radius(double x, double y, double z, double a);
+double ora_hamiltonian(int kerr, const double s[6], double p_t, double M, double a);
+double ora_horizon_radius(double M, double a);
+double ora_photon_orbit_min(double M, double a);
+void ora_isco(double M, double a, double out[3]);
+double ora_emitter_opz(double sx, double sy, double psx, double psy, double E,
+                       double M, double a, double r_isco, double E_isco, double L_isco);
+void ora_camera_basis(const double pos[3], const double target[3], const double up[3],
+                      double fwd[3], double right[3], double upl[3]);
+void ora_camera_tetrad(const double pos[3], const double fwd[3], const double right[3],
+                       const double up[3], double M, double a, const double beta[3],
+                       double u[4], double Ef[4], double Er[4], double Eu[4]);
+void ora_cam_params(const double pos[3], const double target[3], const double up[3],
+                    double fov, const double velocity[3], double M, double a,
+                    double fisheye_deg, double focus_dist, int equirect, double out[49]);
+void ora_init_ray(const ora_scene* sc, int i, int j, double s[6], double* p_t);
+void ora_bardeen(const ora_scene* sc, const double s[6], double p_t, ora_ray* out);
+void ora_blackbody_rgb(double T, double out[3]);
+void ora_blackbody_lut(double wb, double tmin, double tmax, int n, double* out);
+void ora_sample_background(const float* bg, int W, int H, double theta, double phi, double out[3]);
+double ora_sample_volume(const ora_scene* sc, double s, double phi, double z);
+double ora_sim_hash(int x, int y, int z);
+void ora_starfield(const ora_scene* sc, double dx, double dy, double dz, double out[3]);
+
+/* tracers */
+int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* out);
+int ora_trace_tsit5(const ora_scene* sc, int i, int j, double rtol, double atol,
+                    int max_steps, ora_ray* out);
+/* Full-frame render: out[H][W][3] (+= weight * sample).  method 0 = RK4 (same
+ * algorithm as the kernel), 1 = Tsit5 at the given tolerance.  Pixels
+ * (i,j) with i in [0,W), j in [row0,row0+rows).  Returns total steps. */
+long long ora_render(const ora_scene* sc, int method, double tol, int row0, int rows,
+                     float* out, double weight, int nthreads);
+/* Per-pixel records for a list of pixels: rec[n][16] doubles
+ * (fate, steps, dir3, rgb3, ham, r_end, rk_min, ncross, captured0, graze0, rc, Rc). */
+void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const int* ij,
+                    double* rec, int nthreads);
+
+/* Tsitouras 5(4) tableau as used by the oracle: c[7], a[7][7], b[7], btilde[7]. */
+void ora_tsit5_tableau(double c[7], double a[49], double b[7], double btilde[7]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

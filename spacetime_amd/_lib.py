@@ -1,0 +1,143 @@
+"""ctypes binding of libspacetime_hip.so (the C ABI in include/spacetime_hip.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the shared object is missing or was built for another target,
+loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libspacetime_hip.so"
+
+ST_OK, ST_ERR_INVALID_ARG, ST_ERR_OOM, ST_ERR_DEVICE, ST_ERR_LAUNCH, ST_ERR_UNSUPPORTED = range(6)
+ST_PREC_FP32, ST_PREC_MIXED, ST_PREC_FP64 = 0, 1, 2
+REC_N = 12
+REC_FATE, REC_STEPS, REC_DIR, REC_RGB, REC_HAM, REC_REND, REC_FLAGS, REC_RKMIN = 0, 1, 2, 5, 8, 9, 10, 11
+
+
+class StError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libspacetime_hip error {code}: {msg}")
+        self.code = code
+
+
+class TraceArgs(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("nmax", C.c_int),
+        ("dt", C.c_float), ("tol", C.c_float),
+        ("ju", C.c_float), ("jv", C.c_float), ("jw", C.c_float), ("jseed", C.c_int),
+        ("weight", C.c_float),
+        ("row0", C.c_int), ("rows", C.c_int), ("col0", C.c_int), ("cols", C.c_int),
+        ("order", C.c_int),
+        ("h_floor", C.c_float), ("capture_mode", C.c_int), ("capture_factor", C.c_float),
+        ("kerr_shrink", C.c_int), ("esc_interp", C.c_int), ("bardeen", C.c_int),
+        ("precision", C.c_int), ("promote_band", C.c_float), ("promote_horizon", C.c_float),
+        ("stat", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64), ("steps_fp32", C.c_uint64), ("steps_fp64", C.c_uint64),
+        ("promoted", C.c_uint64), ("ms_fp32", C.c_float), ("ms_fp64", C.c_float), ("ms_total", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("out_offset", C.c_int)]
+
+
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol include/spacetime_hip.h declares
+SIGNATURES = {
+    "st_trace_args_default": (None, [C.POINTER(TraceArgs), C.c_int, C.c_int]),
+    "st_ctx_create": (C.c_int, [C.POINTER(C.c_void_p), _F, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "st_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "st_ctx_set_disc": (C.c_int, [C.c_void_p, _F, _F, C.c_int]),
+    "st_ctx_set_disc_temperature": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_int]),
+    "st_ctx_set_volume": (C.c_int, [C.c_void_p, _F, C.c_int, C.c_int, C.c_int, _F]),
+    "st_ctx_set_volume_enabled": (C.c_int, [C.c_void_p, C.c_int]),
+    "st_ctx_set_march_stride": (C.c_int, [C.c_void_p, C.c_int]),
+    "st_ctx_set_starfield": (C.c_int, [C.c_void_p, _F, _F, C.c_int]),
+    "st_ctx_enable_records": (C.c_int, [C.c_void_p, C.c_int]),
+    "st_clear": (C.c_int, [C.c_void_p]),
+    "st_trace": (C.c_int, [C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs)]),
+    "st_sync": (C.c_int, [C.c_void_p]),
+    "st_download": (C.c_int, [C.c_void_p, _F]),
+    "st_download_records": (C.c_int, [C.c_void_p, _F]),
+    "st_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "st_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(FrameDesc), _F, _D, _F]),
+    "st_trace_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),
+    "st_trace_batch_graph": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),
+    "st_download_batch": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_camera_params": (C.c_int, [_D, _D, _D, C.c_double, _D, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, _D]),
+    "st_camera_params_basis": (C.c_int, [_D, _D, _D, _D, C.c_double, _D, C.c_double, C.c_double, _D]),
+    "st_blackbody_lut": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _F]),
+    "st_make_sky": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _F]),
+    "st_make_volume": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _F]),
+    "st_last_error": (C.c_char_p, []),
+    "st_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (once).  Raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m spacetime_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != ST_OK:
+        raise StError(rc, lib().st_last_error().decode())
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_F)
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def device_available() -> bool:
+    """True when a HIP device is visible (counts devices without touching torch)."""
+    try:
+        L = C.CDLL("libamdhip64.so")
+        n = C.c_int(0)
+        return L.hipGetDeviceCount(C.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+def default_args(width: int, height: int) -> TraceArgs:
+    a = TraceArgs()
+    lib().st_trace_args_default(C.byref(a), width, height)
+    return a

@@ -1,0 +1,483 @@
+"""Host-side mirror of SpaceTime.jl's tracer API on the HIP backend.
+
+Names, argument meaning and error behaviour follow the reference:
+  Schwarzschild / Kerr / horizon_radius / photon_orbit_min   src/gr.jl:1-68
+  Camera / Lens / camera_beta                                src/camera.jl:19-50, src/raytrace.jl:22-29
+  Blackbody                                                  src/blackbody.jl:42-62
+  AccretionDisc                                              src/accretion_disc.jl:5-13
+  MetalPreviewContext -> HipPreviewContext                   src/metal.jl:129-182
+  set_* toggles                                              src/metal.jl:192-308
+  _ks_cam_params / _spin_horizon                             src/metal.jl:1739-1774, 347-357
+  render_preview_mtl(!) -> render_preview_hip(!)             src/metal.jl:1790-1881
+  render_draft_mtl -> render_draft_hip                       src/metal.jl:2040-2242
+  render(cam, bh, sky; disc, width, height)                  src/raytrace.jl:339-428
+
+Images are numpy float32 arrays of shape (H, W, 3): the same bytes as the
+reference's Julia (3, W, H) buffer.  Every trace runs on the GPU through
+libspacetime_hip; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, replace
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ---------------------------------------------------------------------------
+# Spacetimes (src/gr.jl)
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class Schwarzschild:
+    M: float = 1.0
+
+
+@dataclass(frozen=True)
+class Kerr:
+    M: float = 1.0
+    a: float = 0.0
+
+    def __post_init__(self):
+        if not abs(self.a) <= self.M:
+            raise ValueError(f"|a| must be <= M (got a={self.a}, M={self.M})")
+
+
+def spin(st) -> float:
+    return float(st.a) if isinstance(st, Kerr) else 0.0
+
+
+def horizon_radius(st) -> float:
+    a = spin(st)
+    return st.M + math.sqrt(max(st.M ** 2 - a * a, 0.0))
+
+
+def photon_orbit_min(st) -> float:
+    a = spin(st)
+    if isinstance(st, Schwarzschild):
+        return 3.0 * st.M
+    return 2.0 * st.M * (1.0 + math.cos((2.0 / 3.0) * math.acos(-a / st.M)))
+
+
+def spin_horizon(st) -> tuple[float, float]:
+    """_spin_horizon (src/metal.jl:347-357): spin and (0.995 r_ph)^2 in f32."""
+    a = np.float32(spin(st))
+    rk = np.float32(0.995 * photon_orbit_min(st))
+    return float(a), float(rk * rk)
+
+
+# ---------------------------------------------------------------------------
+# Camera (src/camera.jl)
+# ---------------------------------------------------------------------------
+def _normalize(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+class Camera:
+    """Pinhole camera: Camera(pos, target, up, fov_factor=1.0; velocity)."""
+
+    def __init__(self, pos, target, up, fov_factor=1.0, velocity=(0.0, 0.0, 0.0)):
+        if isinstance(fov_factor, Lens):
+            fov_factor = fov_factor.fov_factor
+        self.pos = np.asarray(pos, dtype=np.float64)
+        self.target = np.asarray(target, dtype=np.float64)
+        self.up = np.asarray(up, dtype=np.float64)
+        self.fwd = _normalize(self.target - self.pos)
+        self.right = _normalize(np.cross(self.fwd, self.up))
+        self.up_local = np.cross(self.right, self.fwd)
+        self.fov_factor = float(fov_factor)
+        self.velocity = np.asarray(velocity, dtype=np.float64)
+
+
+def camera_beta(cam: Camera) -> np.ndarray:
+    return np.array([cam.velocity @ cam.fwd, cam.velocity @ cam.right, cam.velocity @ cam.up_local])
+
+
+@dataclass(frozen=True)
+class Lens:
+    focal_length: float
+
+    @property
+    def fov_factor(self) -> float:
+        return 18.0 / self.focal_length
+
+
+def ks_cam_params(cam: Camera, M: float, a: float = 0.0, fisheye_deg: float = 0.0, focus_dist: float = 1.0,
+                  equirect: bool = False) -> tuple[np.ndarray, np.ndarray]:
+    """_ks_cam_params (src/metal.jl:1760-1774): (float32[49], float64[49])."""
+    out = np.zeros(49, dtype=np.float64)
+    vel = np.ascontiguousarray(cam.velocity, dtype=np.float64)
+    L.check(L.lib().st_camera_params(
+        L.dptr(np.ascontiguousarray(cam.pos)), L.dptr(np.ascontiguousarray(cam.target)),
+        L.dptr(np.ascontiguousarray(cam.up)), cam.fov_factor, L.dptr(vel), float(M), float(a),
+        float(fisheye_deg), float(focus_dist), int(equirect), L.dptr(out)))
+    return out.astype(np.float32), out
+
+
+def ks_camera_tetrad(pos, fwd, right, up, M, beta=(0.0, 0.0, 0.0), a=0.0):
+    """ks_camera_tetrad (src/preview.jl:93-198) -> (u, Ef, Er, Eu) float64[4] each."""
+    out = np.zeros(49, dtype=np.float64)
+    arr = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+    L.check(L.lib().st_camera_params_basis(L.dptr(arr(pos)), L.dptr(arr(fwd)), L.dptr(arr(right)),
+                                           L.dptr(arr(up)), 1.0, L.dptr(arr(beta)), float(M), float(a),
+                                           L.dptr(out)))
+    return out[16:20].copy(), out[4:8].copy(), out[8:12].copy(), out[12:16].copy()
+
+
+# ---------------------------------------------------------------------------
+# Emission models (src/blackbody.jl, src/accretion_disc.jl, src/disc_volume.jl)
+# ---------------------------------------------------------------------------
+class Blackbody:
+    def __init__(self, wb_temperature=6500.0, table_min=500.0, table_max=30000.0, table_size=1024):
+        self.wb_temperature = float(wb_temperature)
+        self.table_min = float(table_min)
+        self.table_max = float(table_max)
+        self.table_size = int(table_size)
+        self.table = np.zeros((self.table_size, 3), dtype=np.float32)
+        L.check(L.lib().st_blackbody_lut(self.wb_temperature, self.table_min, self.table_max,
+                                         self.table_size, L.fptr(self.table)))
+
+
+REFERENCE_T0 = math.exp(10.034259)   # T_emit = exp(10.034259 - 0.375 ln R^2), R = r/2M
+
+
+@dataclass
+class AccretionDisc:
+    """Thin disc.  The temperature law is T0 (s_T / r0)^-3/4 with
+    s_T = max(s, r_isco) under the ISCO hold; the defaults reproduce the
+    reference law (src/metal.jl:1528-1531): T0 = exp(10.034259), r0 = 2M."""
+    inner_radius: float = 3.0
+    outer_radius: float = 20.0
+    blackbody: Blackbody = field(default_factory=Blackbody)
+    density_falloff: float = 0.0
+    T0: float = REFERENCE_T0
+    r0: Optional[float] = None        # None: 2M
+    hold_isco: bool = True
+
+    def params(self) -> np.ndarray:
+        bb = self.blackbody
+        return np.array([self.inner_radius, self.outer_radius, self.density_falloff, bb.table_min, bb.table_max,
+                         bb.table_size], dtype=np.float32)
+
+
+@dataclass
+class DiscVolume:
+    """Volumetric disc density on a (log s, phi, z) grid.  `density` is
+    indexed [k_z, j_phi, i_r] (the Julia (nr, nphi, nz) column-major bytes)."""
+    density: np.ndarray
+    log_s_in: float
+    log_s_out: float
+    z_max: float
+    emission_scale: float = 1.0
+    opacity_scale: float = 1.0
+    march_stride: int = 2
+
+    def params(self) -> np.ndarray:
+        nz, nphi, nr = self.density.shape
+        return np.array([self.log_s_in, self.log_s_out, self.z_max, nr, nphi, nz, self.emission_scale,
+                         self.opacity_scale, self.march_stride], dtype=np.float32)
+
+
+def make_sky(seed: int = 7, width: int = 1024, height: int = 512, nstars: int = 20000,
+             sigma: float = 1.0) -> np.ndarray:
+    """Seeded synthetic equirectangular sky, float32 (height, width, 3)."""
+    out = np.zeros((height, width, 3), dtype=np.float32)
+    L.check(L.lib().st_make_sky(int(seed), width, height, nstars, float(sigma), L.fptr(out)))
+    return out
+
+
+def make_volume(seed: int, nr: int, nphi: int, nz: int, sigma: float = 1.0) -> np.ndarray:
+    """Seeded log-normal turbulent density, float32 (nz, nphi, nr), unit mean."""
+    out = np.zeros((nz, nphi, nr), dtype=np.float32)
+    L.check(L.lib().st_make_volume(int(seed), nr, nphi, nz, float(sigma), L.fptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Step / termination rule
+# ---------------------------------------------------------------------------
+@dataclass
+class StepRule:
+    """h = dt * min(max(h_coef * r / M, h_floor), h_cap), plus termination.
+
+    `reference()` is the reference's GPU rule (src/metal.jl:1201-1260,
+    2527-2530); `fixed_fraction()` is h = coef * r with escape at a fixed
+    radius and capture at factor * r_plus."""
+    dt: float = 0.02
+    h_coef: float = 0.16
+    h_floor: float = 1.0
+    h_cap: float = 8.0
+    kerr_shrink: bool = True
+    capture_mode: int = 0            # 0 reference kill rules, 1 KS r < capture_factor * r_plus
+    capture_factor: float = 1.01
+    r_escape: Optional[float] = None  # None: r_escape_factor * max(|cam|, 15M)
+    r_escape_factor: float = 2.0
+    nmax: int = 0                    # 0: the reference's budget rule
+    esc_interp: bool = False
+    bardeen: bool = True
+    order: int = 4
+
+    @staticmethod
+    def reference(dt: float = 0.02) -> "StepRule":
+        return StepRule(dt=dt)
+
+    @staticmethod
+    def fixed_fraction(coef: float = 0.05, r_escape: float = 100.0, capture_factor: float = 1.01,
+                       nmax: int = 6000) -> "StepRule":
+        return StepRule(dt=1.0, h_coef=coef, h_floor=0.0, h_cap=1e30, kerr_shrink=False, capture_mode=1,
+                        capture_factor=capture_factor, r_escape=r_escape, nmax=nmax, esc_interp=True)
+
+    def escape_radius(self, cam: Camera, M: float) -> float:
+        if self.r_escape is not None:
+            return float(self.r_escape)
+        return float(np.float32(self.r_escape_factor * max(np.linalg.norm(cam.pos), 15.0 * M)))
+
+
+def spacetime_params(st, cam: Camera, rule: StepRule, relativistic: bool = False) -> np.ndarray:
+    """spacetime_params[8] (src/metal.jl:1850-1854)."""
+    a, rkill2 = spin_horizon(st)
+    return np.array([st.M, 2.05 * st.M, rule.escape_radius(cam, st.M), 1.0 if relativistic else 0.0,
+                     rule.h_coef, rule.h_cap, a, rkill2], dtype=np.float32)
+
+
+def trace_args(width: int, height: int, rule: StepRule, *, precision: int = L.ST_PREC_FP32, ju=0.5, jv=0.5,
+               jw=0.0, jseed=0, weight=1.0, row0=0, rows=None, col0=0, cols=-1,
+               promote_band=0.05, promote_horizon=1.1) -> L.TraceArgs:
+    a = L.default_args(width, height)
+    a.nmax = int(rule.nmax)
+    a.dt = rule.dt
+    a.ju, a.jv, a.jw, a.jseed = ju, jv, jw, jseed
+    a.weight = weight
+    a.row0 = row0
+    a.rows = height - row0 if rows is None else rows
+    a.col0, a.cols = col0, cols
+    a.order = rule.order
+    a.h_floor = rule.h_floor
+    a.capture_mode = rule.capture_mode
+    a.capture_factor = rule.capture_factor
+    a.kerr_shrink = int(rule.kerr_shrink)
+    a.esc_interp = int(rule.esc_interp)
+    a.bardeen = int(rule.bardeen)
+    a.precision = precision
+    a.promote_band = promote_band
+    a.promote_horizon = promote_horizon
+    return a
+
+
+# ---------------------------------------------------------------------------
+# Context (MetalPreviewContext, src/metal.jl:19-182)
+# ---------------------------------------------------------------------------
+class HipPreviewContext:
+    """GPU-resident sky, LUTs, optional gas grid and a (3, W, H) accumulation
+    buffer, plus the trace settings of the preview path."""
+
+    def __init__(self, background: np.ndarray, width: int, height: int, *, dt: float = 0.1, nmax: int = 1000,
+                 r_escape_factor: float = 2.0, disc: Optional[AccretionDisc] = None,
+                 volume: Optional[DiscVolume] = None):
+        bg = np.ascontiguousarray(background, dtype=np.float32)
+        if bg.ndim != 3 or bg.shape[2] != 3:
+            raise ValueError("background must be (height, width, 3)")
+        self.width, self.height = int(width), int(height)
+        self.dt, self.nmax, self.r_escape_factor = float(dt), int(nmax), float(r_escape_factor)
+        h = C.c_void_p()
+        L.check(L.lib().st_ctx_create(C.byref(h), L.fptr(bg), bg.shape[1], bg.shape[0], self.width, self.height))
+        self._h = h
+        self.disc = None
+        self.has_volume = False
+        if disc is not None:
+            self.set_disc(disc)
+        if volume is not None:
+            self.set_volume(volume)
+        self._records = False
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().st_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set_* toggles ------------------------------------------------------
+    def set_disc(self, disc: AccretionDisc, enabled: bool = True):
+        p = disc.params()
+        if not enabled:
+            p[0] = 0.0
+        lut = np.ascontiguousarray(disc.blackbody.table, dtype=np.float32)
+        L.check(L.lib().st_ctx_set_disc(self._h, L.fptr(p), L.fptr(lut), lut.shape[0]))
+        L.check(L.lib().st_ctx_set_disc_temperature(self._h, float(disc.T0), float(disc.r0 or 0.0),
+                                                    int(disc.hold_isco)))
+        self.disc = disc
+
+    def set_disc_enabled(self, disc: AccretionDisc, enabled: bool):
+        self.set_disc(disc, enabled)
+
+    def set_volume(self, vol: DiscVolume):
+        d = np.ascontiguousarray(vol.density, dtype=np.float32)
+        nz, nphi, nr = d.shape
+        L.check(L.lib().st_ctx_set_volume(self._h, L.fptr(d), nr, nphi, nz, L.fptr(vol.params())))
+        self.has_volume = True
+
+    def set_volume_enabled(self, enabled: bool):
+        L.check(L.lib().st_ctx_set_volume_enabled(self._h, int(enabled)))
+
+    def set_march_stride(self, s: int):
+        L.check(L.lib().st_ctx_set_march_stride(self._h, int(s)))
+
+    def set_starfield(self, strength=1.0, texture_weight=0.0, height=None, fov_factor=0.55, density=384, fill=0.5,
+                      flux=0.011, psf_pixels=0.5, galactic=(0.0, 0.0, 1.0), concentration=3.0, temp_min=3000,
+                      temp_max=16000, seed=12345, wb_temperature=10000.0):
+        """set_starfield! (src/metal.jl:257-295)."""
+        H = height if height is not None else self.height
+        g = np.asarray(galactic, dtype=np.float64)
+        gn = np.linalg.norm(g)
+        if not gn > 0:
+            raise ValueError("galactic normal must be non-zero")
+        sigma = psf_pixels * 2 * fov_factor / H
+        bb = Blackbody(wb_temperature=wb_temperature, table_size=1024)
+        sp = np.array([strength, density, fill, sigma, flux, *(g / gn), concentration, temp_min,
+                       temp_max - temp_min, bb.table_min, bb.table_max, bb.table_size, seed, texture_weight],
+                      dtype=np.float32)
+        L.check(L.lib().st_ctx_set_starfield(self._h, L.fptr(sp), L.fptr(bb.table), bb.table_size))
+
+    def enable_records(self, on: bool = True):
+        L.check(L.lib().st_ctx_enable_records(self._h, int(on)))
+        self._records = on
+
+    # -- dispatch -----------------------------------------------------------
+    def clear(self):
+        L.check(L.lib().st_clear(self._h))
+
+    def trace(self, cam32: np.ndarray, cam64: Optional[np.ndarray], sp: np.ndarray, args: L.TraceArgs):
+        """_launch_trace! (src/metal.jl:1888-1933): one asynchronous dispatch."""
+        c32 = np.ascontiguousarray(cam32, dtype=np.float32)
+        c64 = None if cam64 is None else np.ascontiguousarray(cam64, dtype=np.float64)
+        spp = np.ascontiguousarray(sp, dtype=np.float32)
+        L.check(L.lib().st_trace(self._h, L.fptr(c32), L.dptr(c64), L.fptr(spp), C.byref(args)))
+
+    def sync(self):
+        L.check(L.lib().st_sync(self._h))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 3), dtype=np.float32)
+        L.check(L.lib().st_download(self._h, L.fptr(out)))
+        return out
+
+    def records(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, L.REC_N), dtype=np.float32)
+        L.check(L.lib().st_download_records(self._h, L.fptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = L.Stats()
+        L.check(L.lib().st_get_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+
+# reference name kept for callers written against the Metal backend
+MetalPreviewContext = HipPreviewContext
+
+
+def _check_size(ctx: HipPreviewContext, width: int, height: int):
+    if (width, height) != (ctx.width, ctx.height):
+        raise ValueError(f"context is {ctx.width}x{ctx.height}, asked for {width}x{height}")
+
+
+def render_preview_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, fisheye_deg: float = 0.0,
+                       relativistic: bool = False, order: int = 4, tol: float = 1e-4,
+                       rule: Optional[StepRule] = None, precision: int = L.ST_PREC_FP32) -> np.ndarray:
+    """render_preview_mtl (src/metal.jl:1790-1881): one full-frame dispatch of
+    centre-of-pixel rays; returns (H, W, 3) float32."""
+    if rule is None:
+        rule = StepRule(dt=ctx.dt, r_escape_factor=ctx.r_escape_factor, order=order)
+        M = spacetime.M
+        r_esc = rule.escape_radius(cam, M)
+        rule.nmax = int(min(max(ctx.nmax, math.ceil((75.0 + 6.5 * math.log(r_esc / M)) * M / ctx.dt)), 20000))
+    c32, c64 = ks_cam_params(cam, spacetime.M, spin(spacetime), fisheye_deg=fisheye_deg)
+    sp = spacetime_params(spacetime, cam, rule, relativistic)
+    ctx.clear()
+    ctx.trace(c32, c64, sp, trace_args(ctx.width, ctx.height, rule, precision=precision))
+    return ctx.download()
+
+
+def render_preview_hip_(img: np.ndarray, ctx: HipPreviewContext, cam: Camera, spacetime, **kw) -> np.ndarray:
+    """render_preview_mtl! — in-place variant writing into `img` (H, W, 3)."""
+    img[...] = render_preview_hip(ctx, cam, spacetime, **kw)
+    return img
+
+
+def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: int = 1920, height: int = 1080,
+                     samples: int = 2, dt: float = 0.02, rng: Optional[np.random.Generator] = None,
+                     progress=None, fisheye_deg: float = 0.0, aperture_world: float = 0.0,
+                     focus_dist: float = 1.0, relativistic: bool = False, pixel_jitter: bool = True,
+                     order: int = 4, tol: float = 1e-4, rule: Optional[StepRule] = None,
+                     precision: int = L.ST_PREC_FP32) -> np.ndarray:
+    """render_draft_mtl (src/metal.jl:2040-2242): samples^2 stratified passes,
+    per-pixel decorrelated jitter, optional thin-lens strata.  Motion blur
+    and adaptive refinement are not built."""
+    _check_size(ctx, width, height)
+    rng = rng if rng is not None else np.random.default_rng()
+    if rule is None:
+        rule = StepRule.reference(dt)
+    rule = replace(rule, order=order)
+    M = spacetime.M
+    c32, c64 = ks_cam_params(cam, M, spin(spacetime), fisheye_deg=fisheye_deg, focus_dist=focus_dist)
+    sp = spacetime_params(spacetime, cam, rule, relativistic)
+    use_dof = aperture_world > 0.0 and fisheye_deg <= 0.0
+    weight = 1.0 / samples ** 2
+    jw = 1.0 / samples if pixel_jitter else 0.0
+    cells = [(a, b) for b in range(samples) for a in range(samples)]
+    if pixel_jitter:
+        # stratum origins; each pixel hashes its own point inside the stratum
+        offs = [(cells[k][0] / samples, cells[k][1] / samples) for k in rng.permutation(len(cells))]
+    elif samples == 1:
+        offs = [(0.5, 0.5)]
+    else:
+        # jittered_grid: one random point per stratum, shared by the frame
+        offs = [((a + rng.random()) / samples, (b + rng.random()) / samples) for a, b in cells]
+    lens_perm = rng.permutation(samples ** 2)
+    ctx.clear()
+    for p, (du, dv) in enumerate(offs):
+        cp32, cp64 = c32.copy(), c64.copy()
+        if use_dof:
+            m = int(lens_perm[p])
+            for arr in (cp32, cp64):
+                arr[22] = (m // samples) / samples
+                arr[23] = (m % samples) / samples
+                arr[25] = 1.0 / samples
+                arr[26] = aperture_world / 2.0
+                arr[27] = p + 1
+        ctx.trace(cp32, cp64, sp, trace_args(width, height, rule, precision=precision, ju=du, jv=dv, jw=jw,
+                                             jseed=15485863 + p + 1, weight=weight))
+        if progress is not None:
+            ctx.sync()
+            progress((p + 1) / len(offs))
+    return ctx.download()
+
+
+def render(cam: Camera, spacetime, background: np.ndarray, *, disc: Optional[AccretionDisc] = None,
+           volume: Optional[DiscVolume] = None, width: int = 320, height: int = 180, samples: int = 1,
+           relativistic: bool = False, rule: Optional[StepRule] = None, precision: int = L.ST_PREC_MIXED,
+           rng: Optional[np.random.Generator] = None) -> np.ndarray:
+    """render(cam, bh, sky; disc, width, height) (src/raytrace.jl:339-428) on
+    the GPU: FP32 bulk with FP64 promotion by default."""
+    ctx = HipPreviewContext(background, width, height, disc=disc, volume=volume)
+    try:
+        if rule is None:
+            rule = StepRule.reference()
+        return render_draft_hip(ctx, cam, spacetime, width=width, height=height, samples=samples,
+                                relativistic=relativistic, rule=rule, precision=precision, rng=rng,
+                                pixel_jitter=samples > 1)
+    finally:
+        ctx.close()

@@ -1,0 +1,97 @@
+"""Build libspacetime_hip.so for gfx950 in-tree (hipcc, no torch extension).
+
+    python -m spacetime_amd.build            # product library
+    python -m spacetime_amd.build --oracle   # also the CPU oracle (tests only)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+LIB = PKG / "libspacetime_hip.so"
+
+SOURCES = ["st_kernels.hip", "st_api.hip", "st_host.cpp"]
+ARCH = os.environ.get("SPACETIME_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # packed FP32 (v_pk_*_f32) stays off: the compiler would otherwise form
+    # it from ordinary float code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the HIP library cannot be built")
+
+
+def build_library(verbose: bool = False, jobs: int = 4) -> Path:
+    """Compile every source to an object, then link the shared library."""
+    hipcc = _hipcc()
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    procs = []
+    objs = []
+    for src in SOURCES:
+        obj = objdir / (src + ".o")
+        cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(obj)]
+        if src.endswith(".cpp"):
+            cmd = [hipcc, "-x", "c++", "-O2", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+                   str(CSRC / src), "-o", str(obj)]
+        objs.append(obj)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        while len([p for _, p in procs if p.poll() is None]) >= jobs:
+            procs[0][1].wait()
+    failed = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        msgs = [ln for ln in out.splitlines() if "packed-fp32-ops" not in ln]
+        if p.returncode != 0:
+            failed.append((src, "\n".join(msgs)))
+        elif verbose and msgs:
+            print("\n".join(msgs))
+    if failed:
+        raise RuntimeError("\n".join(f"--- {s} ---\n{m}" for s, m in failed))
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    if verbose:
+        print(" ".join(link), flush=True)
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    """Compile the FP64 CPU oracle (test infrastructure) via its Makefile."""
+    oracle_dir = ROOT / "oracle"
+    r = subprocess.run(["make", "-s", "-B", "-C", str(oracle_dir)], capture_output=True, text=True)
+    if verbose and (r.stdout or r.stderr):
+        print(r.stdout + r.stderr)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+    return oracle_dir / "liboracle.so"
+
+
+if __name__ == "__main__":
+    v = "-v" in sys.argv
+    print(build_library(verbose=v))
+    if "--oracle" in sys.argv:
+        print(build_oracle(verbose=v))

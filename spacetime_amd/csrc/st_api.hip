@@ -1,0 +1,574 @@
+// C-ABI of libspacetime_hip: context management, trace dispatch, batched
+// frames, statistics.  Declarations and reference citations live in
+// include/spacetime_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "spacetime_hip.h"
+#include "st_internal.h"
+#include "st_trace.h"
+
+using st::DeviceScene;
+using st::FrameRef;
+using st::TraceParams;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define ST_HIP(call)                                                                                        \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? ST_ERR_OOM : ST_ERR_DEVICE,                             \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                                 \
+    } while (0)
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int tw = 0, nblocks = 0;
+    st_trace_args args{};
+    bool valid = false;
+};
+
+}  // namespace
+
+struct st_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    int width = 0, height = 0;
+    float4* sky = nullptr;
+    int sky_w = 0, sky_h = 0;
+    float4* lut = nullptr;
+    int lut_n = 1;
+    float disc_params[6] = {0, 0, 0, 500.f, 30000.f, 1.f};
+    float T0 = 22800.0f, Tr0 = 2.0f;
+    int T_hold = 1;
+    bool T_from_M = true;   // reference law: r0 = 2M unless set explicitly
+    float* vol = nullptr;
+    int nr = 2, nphi = 1, nz = 2;
+    float vol_params[9] = {0, 0, 0, 0, 0, 0, 0, 0, 2.0f};
+    bool has_volume = false, vol_on = false;
+    float4* star_lut = nullptr;
+    int star_lut_n = 1;
+    float star_params[16] = {0};
+    float* out = nullptr;
+    float* rec = nullptr;
+    bool rec_on = false;
+    uint2* queue = nullptr;
+    size_t queue_cap = 0;
+    unsigned* qcount = nullptr;
+    unsigned long long* counters = nullptr;
+    unsigned* work_next = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool last_mixed = false;
+    bool have_last = false;
+    // batch
+    int nframes = 0;
+    std::vector<FrameRef> frames_host;
+    FrameRef* frames_dev = nullptr;
+    float* batch_out = nullptr;
+    size_t batch_out_n = 0;
+    int* tile_base_dev = nullptr;
+    int tile_base_tw = 0;
+    int total_items = 0;
+    float batch_st[8] = {0};
+    GraphCache graph;
+};
+
+namespace {
+
+void free_dev(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int upload_float4(float4** dst, const float* src3, int n) {
+    std::vector<float4> h((size_t)std::max(n, 1));
+    for (int k = 0; k < n; ++k) h[k] = make_float4(src3[3 * k], src3[3 * k + 1], src3[3 * k + 2], 0.0f);
+    if (n <= 0) h[0] = make_float4(0, 0, 0, 0);
+    float4* d = nullptr;
+    ST_HIP(hipMalloc(&d, sizeof(float4) * h.size()));
+    ST_HIP(hipMemcpy(d, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice));
+    free_dev(*dst);
+    *dst = d;
+    return ST_OK;
+}
+
+DeviceScene make_scene(const st_ctx* c) {
+    DeviceScene D;
+    D.sky = c->sky; D.sky_w = c->sky_w; D.sky_h = c->sky_h;
+    D.lut = c->lut; D.lut_n = std::max(c->lut_n, 1);
+    D.lut_tmin = c->disc_params[3]; D.lut_tmax = c->disc_params[4];
+    D.disc_on = c->disc_params[0] > 0.0f && c->disc_params[1] > c->disc_params[0] && c->lut_n > 0;
+    D.disc_inner = c->disc_params[0]; D.disc_outer = c->disc_params[1]; D.disc_falloff = c->disc_params[2];
+    D.T0 = c->T0; D.Tr0 = c->Tr0; D.T_hold = c->T_hold;
+    D.vol = c->vol; D.nr = c->nr; D.nphi = c->nphi; D.nz = c->nz;
+    std::memcpy(D.vol_params, c->vol_params, sizeof(D.vol_params));
+    D.star_lut = c->star_lut; D.star_lut_n = std::max(c->star_lut_n, 1);
+    std::memcpy(D.star_params, c->star_params, sizeof(D.star_params));
+    if (!c->star_lut) D.star_params[0] = 0.0f;
+    return D;
+}
+
+int ref_nmax(float r_escape, float M, float dt) {
+    const double n = std::ceil((75.0 + 6.5 * std::log(std::max((double)r_escape / M, 1.0))) * M / dt);
+    return (int)std::min(std::max(1000.0, n), 40000.0);
+}
+
+int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P) {
+    if (!(sp[0] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "spacetime_params[0] (M) must be > 0");
+    if (std::fabs(sp[6]) > sp[0] * (1.0f + 1e-6f)) return fail(ST_ERR_INVALID_ARG, "|a| must be <= M");
+    if (!(a->dt > 0.0f)) return fail(ST_ERR_INVALID_ARG, "dt must be > 0");
+    if (a->order != 4 && a->order != 2 && a->order != 1)
+        return fail(ST_ERR_UNSUPPORTED, "order must be 4, 2 or 1 (adaptive 45/46 not built)");
+    if (a->stat) return fail(ST_ERR_UNSUPPORTED, "STAT refinement variant not built");
+    if (a->precision < 0 || a->precision > 2) return fail(ST_ERR_INVALID_ARG, "precision must be 0, 1 or 2");
+    std::memset(&P, 0, sizeof(P));
+    P.M = sp[0]; P.r_escape = sp[2]; P.relativistic = sp[3] > 0.5f;
+    P.h_coef = sp[4]; P.h_cap = sp[5]; P.a = sp[6]; P.rkill2 = sp[7];
+    P.nmax = a->nmax > 0 ? a->nmax : ref_nmax(sp[2], sp[0], a->dt);
+    P.dt = a->dt; P.ju = a->ju; P.jv = a->jv; P.jw = a->jw; P.jseed = a->jseed;
+    P.weight = a->weight; P.order = a->order; P.h_floor = a->h_floor;
+    P.capture_mode = a->capture_mode; P.capture_factor = a->capture_factor;
+    P.kerr_shrink = a->kerr_shrink; P.esc_interp = a->esc_interp; P.bardeen = a->bardeen;
+    P.promote_band = a->promote_band; P.promote_horizon = a->promote_horizon;
+    P.assign = 0;
+    P.queue = c->queue; P.qcount = c->qcount; P.counters = c->counters;
+    return ST_OK;
+}
+
+int ensure_queue(st_ctx* c, size_t n) {
+    if (c->queue_cap >= n) return ST_OK;
+    free_dev(c->queue);
+    c->queue = nullptr;
+    ST_HIP(hipMalloc(&c->queue, sizeof(uint2) * std::max<size_t>(n, 1)));
+    c->queue_cap = n;
+    return ST_OK;
+}
+
+void sync_temperature_scale(st_ctx* c, float M) {
+    if (c->T_from_M) c->Tr0 = 2.0f * M;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* st_last_error(void) { return g_err.c_str(); }
+const char* st_version(void) { return "libspacetime_hip 0.1 (gfx950)"; }
+
+void st_trace_args_default(st_trace_args* a, int width, int height) {
+    std::memset(a, 0, sizeof(*a));
+    a->width = width; a->height = height;
+    a->nmax = 0;            // reference rule from r_escape and dt
+    a->dt = 0.02f;          // render_draft_mtl default (src/metal.jl:2043)
+    a->tol = 1e-4f;
+    a->ju = 0.5f; a->jv = 0.5f; a->jw = 0.0f; a->jseed = 0;
+    a->weight = 1.0f;
+    a->row0 = 0; a->rows = height; a->col0 = 0; a->cols = -1;
+    a->order = 4;
+    a->h_floor = 1.0f;
+    a->capture_mode = 0;
+    a->capture_factor = 1.01f;
+    a->kerr_shrink = 1;
+    a->esc_interp = 0;
+    a->bardeen = 1;
+    a->precision = ST_PREC_FP32;
+    a->promote_band = 0.05f;
+    a->promote_horizon = 1.1f;
+    a->stat = 0;
+}
+
+int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, int height) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!bg || bg_w < 2 || bg_h < 2) return fail(ST_ERR_INVALID_ARG, "background must be at least 2x2");
+    if (width <= 0 || height <= 0) return fail(ST_ERR_INVALID_ARG, "width and height must be > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ST_ERR_DEVICE, "no HIP device");
+    st_ctx* c = new st_ctx();
+    ST_HIP(hipGetDevice(&c->device));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+        c->num_cu = ncu;
+    c->width = width; c->height = height;
+    int rc;
+    if ((rc = upload_float4(&c->sky, bg, bg_w * bg_h)) != ST_OK) { st_ctx_destroy(c); return rc; }
+    c->sky_w = bg_w; c->sky_h = bg_h;
+    const float zero3[3] = {0, 0, 0};
+    if ((rc = upload_float4(&c->lut, zero3, 1)) != ST_OK) { st_ctx_destroy(c); return rc; }
+    c->lut_n = 1;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->out, sizeof(float) * 3 * (size_t)width * height);
+    if (e == hipSuccess) e = hipMemset(c->out, 0, sizeof(float) * 3 * (size_t)width * height);
+    if (e == hipSuccess) e = hipMalloc(&c->qcount, sizeof(unsigned) * 4);
+    if (e == hipSuccess) e = hipMalloc(&c->work_next, sizeof(unsigned) * 4);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(unsigned long long) * 4);
+    if (e == hipSuccess) e = hipMalloc(&c->vol, sizeof(float) * 4);
+    if (e == hipSuccess) e = hipMemset(c->vol, 0, sizeof(float) * 4);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e != hipSuccess) {
+        st_ctx_destroy(c);
+        return fail(e == hipErrorOutOfMemory ? ST_ERR_OOM : ST_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if ((rc = ensure_queue(c, (size_t)width * height)) != ST_OK) { st_ctx_destroy(c); return rc; }
+    *out = c;
+    return ST_OK;
+}
+
+int st_ctx_destroy(st_ctx* c) {
+    if (!c) return ST_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
+    free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
+    free_dev(c->out); free_dev(c->rec); free_dev(c->queue); free_dev(c->qcount);
+    free_dev(c->counters); free_dev(c->work_next); free_dev(c->frames_dev);
+    free_dev(c->batch_out); free_dev(c->tile_base_dev);
+    for (int k = 0; k < 3; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ST_OK;
+}
+
+int st_ctx_set_disc(st_ctx* c, const float dp[6], const float* lut, int n) {
+    if (!c || !dp) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (dp[0] > 0.0f) {
+        if (!lut || n < 1) return fail(ST_ERR_INVALID_ARG, "an enabled disc needs a LUT");
+        if (!(dp[4] > dp[3])) return fail(ST_ERR_INVALID_ARG, "LUT T_max must exceed T_min");
+        if ((int)dp[5] != n) return fail(ST_ERR_INVALID_ARG, "disc_params[5] must equal the LUT size");
+    }
+    if (lut && n >= 1) {
+        const int rc = upload_float4(&c->lut, lut, n);
+        if (rc != ST_OK) return rc;
+        c->lut_n = n;
+    }
+    std::memcpy(c->disc_params, dp, sizeof(c->disc_params));
+    return ST_OK;
+}
+
+int st_ctx_set_disc_temperature(st_ctx* c, float T0, float r0, int hold_isco) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (!(T0 > 0.0f)) return fail(ST_ERR_INVALID_ARG, "T0 must be > 0");
+    c->T0 = T0;
+    if (r0 > 0.0f) { c->Tr0 = r0; c->T_from_M = false; }
+    else c->T_from_M = true;
+    c->T_hold = hold_isco ? 1 : 0;
+    return ST_OK;
+}
+
+int st_ctx_set_volume(st_ctx* c, const float* vol, int nr, int nphi, int nz, const float vp[9]) {
+    if (!c || !vol || !vp) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (nr < 2 || nphi < 1 || nz < 2) return fail(ST_ERR_INVALID_ARG, "volume must be at least 2x1x2");
+    if (!(vp[1] > vp[0]) || !(vp[2] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "bad vol_params");
+    const size_t n = (size_t)nr * nphi * nz;
+    float* d = nullptr;
+    ST_HIP(hipMalloc(&d, sizeof(float) * n));
+    ST_HIP(hipMemcpy(d, vol, sizeof(float) * n, hipMemcpyHostToDevice));
+    free_dev(c->vol);
+    c->vol = d;
+    c->nr = nr; c->nphi = nphi; c->nz = nz;
+    std::memcpy(c->vol_params, vp, sizeof(c->vol_params));
+    c->vol_params[3] = (float)nr; c->vol_params[4] = (float)nphi; c->vol_params[5] = (float)nz;
+    c->has_volume = true;
+    c->vol_on = true;
+    return ST_OK;
+}
+
+int st_ctx_set_volume_enabled(st_ctx* c, int enabled) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (!c->has_volume) return ST_OK;   // no-op without a volume, as the reference
+    c->vol_on = enabled != 0;
+    return ST_OK;
+}
+
+int st_ctx_set_march_stride(st_ctx* c, int stride) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (!c->has_volume) return ST_OK;
+    c->vol_params[8] = (float)stride;
+    return ST_OK;
+}
+
+int st_ctx_set_starfield(st_ctx* c, const float sp[16], const float* lut, int n) {
+    if (!c || !sp) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (sp[0] > 0.0f) {
+        if (!lut || n < 1) return fail(ST_ERR_INVALID_ARG, "enabled stars need a LUT");
+        if (!(sp[1] >= 1.0f) || !(sp[3] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "bad star_params");
+    }
+    if (lut && n >= 1) {
+        const int rc = upload_float4(&c->star_lut, lut, n);
+        if (rc != ST_OK) return rc;
+        c->star_lut_n = n;
+    }
+    std::memcpy(c->star_params, sp, sizeof(c->star_params));
+    return ST_OK;
+}
+
+int st_ctx_enable_records(st_ctx* c, int on) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    c->rec_on = on != 0;
+    if (c->rec_on && !c->rec) {
+        const size_t n = (size_t)c->width * c->height * ST_REC_N;
+        ST_HIP(hipMalloc(&c->rec, sizeof(float) * n));
+        ST_HIP(hipMemset(c->rec, 0, sizeof(float) * n));
+    }
+    return ST_OK;
+}
+
+int st_clear(st_ctx* c) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    ST_HIP(hipMemsetAsync(c->out, 0, sizeof(float) * 3 * (size_t)c->width * c->height, c->stream));
+    return ST_OK;
+}
+
+int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp[8], const st_trace_args* a) {
+    if (!c || !cam || !sp || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (a->width != c->width || a->height != c->height)
+        return fail(ST_ERR_INVALID_ARG, "args width/height must match the context");
+    const int cols = a->cols < 0 ? c->width : a->cols;
+    if (a->row0 < 0 || a->rows < 0 || a->row0 + a->rows > c->height || a->col0 < 0 || a->col0 + cols > c->width)
+        return fail(ST_ERR_INVALID_ARG, "tile outside the frame");
+    TraceParams P;
+    int rc = fill_params(c, sp, a, P);
+    if (rc != ST_OK) return rc;
+    sync_temperature_scale(c, sp[0]);
+    DeviceScene D = make_scene(c);
+    FrameRef F;
+    std::memset(&F, 0, sizeof(F));
+    std::memcpy(F.c32, cam, sizeof(F.c32));
+    if (cam64) { std::memcpy(F.c64, cam64, sizeof(F.c64)); F.has64 = 1; }
+    F.width = c->width; F.height = c->height;
+    F.out = c->out;
+    F.rec = c->rec_on ? c->rec : nullptr;
+    const bool kerr = sp[6] != 0.0f;
+    const bool vol = c->has_volume && c->vol_on;
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
+    const int tw = 8;
+    if (a->precision == ST_PREC_MIXED) {
+        ST_HIP(st::launch_tiles(0, kerr, vol, true, D, P, F, a->col0, a->row0, cols, a->rows, tw, c->stream));
+        ST_HIP(hipEventRecord(c->ev[1], c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, D, P, F, nullptr, c->num_cu * 2, c->stream));
+    } else {
+        ST_HIP(st::launch_tiles(a->precision == ST_PREC_FP64, kerr, vol, false, D, P, F, a->col0, a->row0, cols,
+                                a->rows, tw, c->stream));
+        ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    }
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = a->precision == ST_PREC_MIXED;
+    c->have_last = true;
+    return ST_OK;
+}
+
+int st_sync(st_ctx* c) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_download(st_ctx* c, float* host) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    ST_HIP(hipMemcpyAsync(host, c->out, sizeof(float) * 3 * (size_t)c->width * c->height, hipMemcpyDeviceToHost,
+                          c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_download_records(st_ctx* c, float* host) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->rec) return fail(ST_ERR_INVALID_ARG, "records are not enabled");
+    ST_HIP(hipMemcpyAsync(host, c->rec, sizeof(float) * ST_REC_N * (size_t)c->width * c->height,
+                          hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_get_stats(st_ctx* c, st_stats* s) {
+    if (!c || !s) return fail(ST_ERR_INVALID_ARG, "null argument");
+    std::memset(s, 0, sizeof(*s));
+    if (!c->have_last) return ST_OK;
+    ST_HIP(hipStreamSynchronize(c->stream));
+    unsigned long long cnt[4];
+    unsigned q = 0;
+    ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    ST_HIP(hipMemcpy(&q, c->qcount, sizeof(q), hipMemcpyDeviceToHost));
+    s->rays = cnt[3];
+    s->steps_fp32 = cnt[0];
+    s->steps_fp64 = cnt[1];
+    s->promoted = c->last_mixed ? q : 0;
+    float ms = 0.0f;
+    ST_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    s->ms_fp32 = ms;
+    ST_HIP(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    s->ms_fp64 = ms;
+    ST_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[2]));
+    s->ms_total = ms;
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched frames
+// ---------------------------------------------------------------------------
+int st_batch_create(st_ctx* c, int nframes, const st_frame_desc* fr, const float* cam, const double* cam64,
+                    const float sp[8]) {
+    if (!c || !fr || !cam || !sp || nframes <= 0) return fail(ST_ERR_INVALID_ARG, "bad batch arguments");
+    size_t total = 0, npix = 0;
+    for (int k = 0; k < nframes; ++k) {
+        if (fr[k].width <= 0 || fr[k].height <= 0 || fr[k].out_offset < 0)
+            return fail(ST_ERR_INVALID_ARG, "bad frame descriptor");
+        total = std::max(total, (size_t)fr[k].out_offset + 3 * (size_t)fr[k].width * fr[k].height);
+        npix += (size_t)fr[k].width * fr[k].height;
+    }
+    if (c->graph.exec) { (void)hipGraphExecDestroy(c->graph.exec); c->graph.exec = nullptr; }
+    if (c->graph.graph) { (void)hipGraphDestroy(c->graph.graph); c->graph.graph = nullptr; }
+    c->graph.valid = false;
+    free_dev(c->batch_out); c->batch_out = nullptr;
+    free_dev(c->frames_dev); c->frames_dev = nullptr;
+    ST_HIP(hipMalloc(&c->batch_out, sizeof(float) * std::max<size_t>(total, 1)));
+    ST_HIP(hipMemset(c->batch_out, 0, sizeof(float) * std::max<size_t>(total, 1)));
+    c->batch_out_n = total;
+    c->frames_host.assign(nframes, FrameRef{});
+    for (int k = 0; k < nframes; ++k) {
+        FrameRef& F = c->frames_host[k];
+        std::memset(&F, 0, sizeof(F));
+        std::memcpy(F.c32, cam + 49 * (size_t)k, sizeof(F.c32));
+        if (cam64) { std::memcpy(F.c64, cam64 + 49 * (size_t)k, sizeof(F.c64)); F.has64 = 1; }
+        F.width = fr[k].width; F.height = fr[k].height;
+        F.out = c->batch_out + fr[k].out_offset;
+        F.rec = nullptr;
+    }
+    ST_HIP(hipMalloc(&c->frames_dev, sizeof(FrameRef) * nframes));
+    ST_HIP(hipMemcpy(c->frames_dev, c->frames_host.data(), sizeof(FrameRef) * nframes, hipMemcpyHostToDevice));
+    c->nframes = nframes;
+    std::memcpy(c->batch_st, sp, sizeof(c->batch_st));
+    c->tile_base_tw = 0;
+    const int rc = ensure_queue(c, npix);
+    if (rc != ST_OK) return rc;
+    return ST_OK;
+}
+
+namespace {
+
+int prepare_tiles(st_ctx* c, int tw) {
+    if (c->tile_base_tw == tw && c->tile_base_dev) return ST_OK;
+    std::vector<int> base(c->nframes);
+    int acc = 0;
+    const int th = 64 / tw;
+    for (int k = 0; k < c->nframes; ++k) {
+        base[k] = acc;
+        const FrameRef& F = c->frames_host[k];
+        acc += ((F.width + tw - 1) / tw) * ((F.height + th - 1) / th);
+    }
+    free_dev(c->tile_base_dev);
+    c->tile_base_dev = nullptr;
+    ST_HIP(hipMalloc(&c->tile_base_dev, sizeof(int) * c->nframes));
+    ST_HIP(hipMemcpy(c->tile_base_dev, base.data(), sizeof(int) * c->nframes, hipMemcpyHostToDevice));
+    c->tile_base_tw = tw;
+    c->total_items = acc;
+    return ST_OK;
+}
+
+int enqueue_batch(st_ctx* c, const st_trace_args* a, int tw, int nblocks) {
+    TraceParams P;
+    int rc = fill_params(c, c->batch_st, a, P);
+    if (rc != ST_OK) return rc;
+    P.assign = 1;
+    sync_temperature_scale(c, c->batch_st[0]);
+    DeviceScene D = make_scene(c);
+    const bool kerr = c->batch_st[6] != 0.0f;
+    const bool vol = c->has_volume && c->vol_on;
+    const bool route = a->precision == ST_PREC_MIXED;
+    if (a->precision == ST_PREC_FP64) return fail(ST_ERR_UNSUPPORTED, "batched FP64-only tracing is not built");
+    ST_HIP(st::launch_reset(c->qcount, c->counters, c->work_next, c->stream));
+    ST_HIP(st::launch_batch(kerr, vol, route, D, P, c->frames_dev, c->tile_base_dev, c->nframes, c->total_items,
+                            tw, c->work_next, nblocks, c->stream));
+    if (route) {
+        FrameRef dummy;
+        std::memset(&dummy, 0, sizeof(dummy));
+        ST_HIP(st::launch_queue64(kerr, vol, D, P, dummy, c->frames_dev, c->num_cu * 2, c->stream));
+    }
+    return ST_OK;
+}
+
+void resolve_shape(st_ctx* c, int& tw, int& waves) {
+    if (tw != 8 && tw != 16 && tw != 32 && tw != 64) tw = 8;
+    if (waves <= 0) waves = 8;
+    (void)c;
+}
+
+}  // namespace
+
+int st_trace_batch(st_ctx* c, const st_trace_args* a, int tile, int waves_per_cu) {
+    if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
+    int tw = tile, waves = waves_per_cu;
+    resolve_shape(c, tw, waves);
+    int rc = prepare_tiles(c, tw);
+    if (rc != ST_OK) return rc;
+    const int nblocks = std::max(1, c->num_cu * waves / 4);
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    rc = enqueue_batch(c, a, tw, nblocks);
+    if (rc != ST_OK) return rc;
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = a->precision == ST_PREC_MIXED;
+    c->have_last = true;
+    return ST_OK;
+}
+
+int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_per_cu) {
+    if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
+    int tw = tile, waves = waves_per_cu;
+    resolve_shape(c, tw, waves);
+    int rc = prepare_tiles(c, tw);
+    if (rc != ST_OK) return rc;
+    const int nblocks = std::max(1, c->num_cu * waves / 4);
+    GraphCache& g = c->graph;
+    const bool same = g.valid && g.tw == tw && g.nblocks == nblocks && std::memcmp(&g.args, a, sizeof(*a)) == 0;
+    if (!same) {
+        if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+        if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }
+        g.valid = false;
+        ST_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_batch(c, a, tw, nblocks);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != ST_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e != hipSuccess) return fail(ST_ERR_LAUNCH, std::string("graph capture: ") + hipGetErrorString(e));
+        ST_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+        g.graph = graph;
+        g.tw = tw; g.nblocks = nblocks; g.args = *a; g.valid = true;
+    }
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(hipGraphLaunch(g.exec, c->stream));
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = a->precision == ST_PREC_MIXED;
+    c->have_last = true;
+    return ST_OK;
+}
+
+int st_download_batch(st_ctx* c, float* host, size_t n) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (n > c->batch_out_n) return fail(ST_ERR_INVALID_ARG, "more floats requested than the batch holds");
+    ST_HIP(hipMemcpyAsync(host, c->batch_out, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+// Trace kernels for gfx950 and their launchers.
+//
+//  k_trace_tiles    one ray per lane; each 64-lane wave owns a compact pixel
+//                   tile (8x8 by default) so lanes of a wave take similar
+//                   step counts.  FP32 (optionally routing promoted rays to
+//                   the FP64 queue) or FP64.
+//  k_trace_queue64  FP64 re-trace of the compacted promoted-ray queue,
+//                   grid-stride over a device-side count (no host sync).
+//  k_trace_batch    persistent kernel over a global work queue of pixel
+//                   tiles spanning many frames of uneven size.
+#include <hip/hip_runtime.h>
+
+#include "st_internal.h"
+#include "st_trace.h"
+
+namespace st {
+
+// Tile geometry of a launch: wave tile TW x (64/TW), 4 waves per block
+// stacked vertically.
+struct TileGeom {
+    int col0, row0, cols, rows;
+    int tw;          // wave tile width (8, 16, 32 or 64)
+    int tiles_x;     // block tiles across
+};
+
+__device__ __forceinline__ void tile_pixel(const TileGeom& G, int block, int tid, int& i, int& j, bool& valid) {
+    const int lane = tid & 63, wv = tid >> 6;
+    const int th = 64 / G.tw;               // wave tile height
+    const int bx = block % G.tiles_x, by = block / G.tiles_x;
+    const int li = lane % G.tw, lj = lane / G.tw;
+    i = G.col0 + bx * G.tw + li;
+    j = G.row0 + by * (4 * th) + wv * th + lj;
+    valid = (i < G.col0 + G.cols) && (j < G.row0 + G.rows);
+}
+
+template <typename T, bool KERR, bool VOL, bool ROUTE>
+__global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams P, FrameRef F, TileGeom G) {
+    int i, j;
+    bool valid;
+    tile_pixel(G, blockIdx.x, threadIdx.x, i, j, valid);
+    unsigned long long steps = 0;
+    bool push = false;
+    if (valid) {
+        RayOut<T> o;
+        trace_ray<T, KERR, VOL, ROUTE>(D, P, F, i, j, o);
+        if (ROUTE && o.routed) {
+            push = true;
+        } else {
+            store_ray(P, F, i, j, o);
+        }
+        steps = (unsigned long long)o.steps;
+    }
+    if constexpr (ROUTE) {
+        // wave-aggregated append of promoted rays to the FP64 queue
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(P.qcount, (unsigned)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (push) {
+                const unsigned off = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+                P.queue[base + off] = make_uint2(0u, (unsigned)(j * F.width + i));
+            }
+        }
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+    }
+}
+
+// FP64 re-trace of promoted rays.  BATCH: a queue entry's frame index
+// selects frames[]; otherwise every entry belongs to the single frame F0.
+template <bool KERR, bool VOL, bool BATCH>
+__global__ __launch_bounds__(256) void k_trace_queue64(DeviceScene D, TraceParams P, FrameRef F0,
+                                                       const FrameRef* frames) {
+    const unsigned n = *P.qcount;
+    unsigned long long steps = 0;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const uint2 e = P.queue[q];
+        const FrameRef& F = BATCH ? frames[e.x] : F0;
+        const int i = (int)(e.y % (unsigned)F.width), j = (int)(e.y / (unsigned)F.width);
+        RayOut<double> o;
+        trace_ray<double, KERR, VOL, false>(D, P, F, i, j, o);
+        store_ray(P, F, i, j, o);
+        steps += (unsigned long long)o.steps;
+    }
+    const unsigned long long tot = wave_sum(steps);
+    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&P.counters[1], tot);
+}
+
+__global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters, unsigned* work_next) {
+    if (threadIdx.x == 0) {
+        *qcount = 0u;
+        if (work_next) *work_next = 0u;
+    }
+    if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
+}
+
+// Persistent batched tracer.  Work item w is one 64-pixel wave tile of some
+// frame; tile_base[k] is the first work item of frame k (prefix sum).  Each
+// wave takes items from a global counter until the queue is drained, so
+// frames of different sizes and step counts load-balance across the chip.
+template <bool KERR, bool VOL, bool ROUTE>
+__global__ __launch_bounds__(256) void k_trace_batch(DeviceScene D, TraceParams P, const FrameRef* frames,
+                                                     const int* tile_base, int nframes, int total_items,
+                                                     int tw, unsigned* work_next) {
+    const int lane = threadIdx.x & 63;
+    const int th = 64 / tw;
+    unsigned long long steps = 0, nrays = 0;
+    for (;;) {
+        unsigned item = 0;
+        if (lane == 0) item = atomicAdd(work_next, 1u);
+        item = __shfl(item, 0, 64);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= (unsigned)total_items) break;
+        // frame of this item: binary search of the prefix sums
+        int lo = 0, hi = nframes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_base[mid] <= (int)item) lo = mid; else hi = mid - 1;
+        }
+        const FrameRef& F = frames[lo];
+        const int local = (int)item - tile_base[lo];
+        const int tiles_x = (F.width + tw - 1) / tw;
+        const int tx = local % tiles_x, ty = local / tiles_x;
+        const int i = tx * tw + lane % tw, j = ty * th + lane / tw;
+        const bool valid = i < F.width && j < F.height;
+        bool push = false;
+        if (valid) {
+            RayOut<float> o;
+            trace_ray<float, KERR, VOL, ROUTE>(D, P, F, i, j, o);
+            if (ROUTE && o.routed) push = true;
+            else store_ray(P, F, i, j, o);
+            steps += (unsigned long long)o.steps;
+            nrays += 1;
+        }
+        if constexpr (ROUTE) {
+            const unsigned long long mask = __ballot(push);
+            if (mask) {
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(P.qcount, (unsigned)__popcll(mask));
+                base = __shfl(base, leader, 64);
+                if (push) {
+                    const unsigned off = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+                    P.queue[base + off] = make_uint2((unsigned)lo, (unsigned)(j * F.width + i));
+                }
+            }
+        }
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(nrays);
+    if (lane == 0) {
+        if (tot) atomicAdd(&P.counters[0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <typename T, bool KERR, bool VOL, bool ROUTE>
+static hipError_t launch_tiles_t(const DeviceScene& D, const TraceParams& P, const FrameRef& F, const TileGeom& G,
+                                 int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_tiles<T, KERR, VOL, ROUTE>), dim3(nblocks), dim3(256), 0, s, D, P, F, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
+                        const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s) {
+    TileGeom G;
+    G.col0 = col0; G.row0 = row0; G.cols = cols; G.rows = rows; G.tw = tw;
+    const int th = 64 / tw;
+    G.tiles_x = (cols + tw - 1) / tw;
+    const int tiles_y = (rows + 4 * th - 1) / (4 * th);
+    const int nblocks = G.tiles_x * tiles_y;
+    if (nblocks <= 0) return hipSuccess;
+#define ST_DISPATCH(TT, KK, VV, RR) return launch_tiles_t<TT, KK, VV, RR>(D, P, F, G, nblocks, s)
+    if (fp64) {
+        if (kerr) { if (vol) ST_DISPATCH(double, true, true, false); else ST_DISPATCH(double, true, false, false); }
+        else { if (vol) ST_DISPATCH(double, false, true, false); else ST_DISPATCH(double, false, false, false); }
+    }
+    if (kerr) {
+        if (vol) { if (route) ST_DISPATCH(float, true, true, true); else ST_DISPATCH(float, true, true, false); }
+        else { if (route) ST_DISPATCH(float, true, false, true); else ST_DISPATCH(float, true, false, false); }
+    } else {
+        if (vol) { if (route) ST_DISPATCH(float, false, true, true); else ST_DISPATCH(float, false, true, false); }
+        else { if (route) ST_DISPATCH(float, false, false, true); else ST_DISPATCH(float, false, false, false); }
+    }
+#undef ST_DISPATCH
+}
+
+hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
+                          const FrameRef* frames_dev, int nblocks, hipStream_t s) {
+#define ST_Q(KK, VV, BB) \
+    hipLaunchKernelGGL((k_trace_queue64<KK, VV, BB>), dim3(nblocks), dim3(256), 0, s, D, P, F0, frames_dev)
+    const bool batch = frames_dev != nullptr;
+    if (kerr) {
+        if (vol) { if (batch) ST_Q(true, true, true); else ST_Q(true, true, false); }
+        else { if (batch) ST_Q(true, false, true); else ST_Q(true, false, false); }
+    } else {
+        if (vol) { if (batch) ST_Q(false, true, true); else ST_Q(false, true, false); }
+        else { if (batch) ST_Q(false, false, true); else ST_Q(false, false, false); }
+    }
+#undef ST_Q
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, qcount, counters, work_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
+                        const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,
+                        unsigned* work_next, int nblocks, hipStream_t s) {
+#define ST_B(KK, VV, RR)                                                                                       \
+    hipLaunchKernelGGL((k_trace_batch<KK, VV, RR>), dim3(nblocks), dim3(256), 0, s, D, P, frames_dev,          \
+                       tile_base_dev, nframes, total_items, tw, work_next)
+    if (kerr) {
+        if (vol) { if (route) ST_B(true, true, true); else ST_B(true, true, false); }
+        else { if (route) ST_B(true, false, true); else ST_B(true, false, false); }
+    } else {
+        if (vol) { if (route) ST_B(false, true, true); else ST_B(false, true, false); }
+        else { if (route) ST_B(false, false, true); else ST_B(false, false, false); }
+    }
+#undef ST_B
+    return hipGetLastError();
+}
+
+}  // namespace st

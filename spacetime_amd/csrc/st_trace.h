@@ -1,0 +1,687 @@
+// Per-ray tracer shared by every kernel variant (FP32 bulk, FP64 promoted,
+// batched).  One ray per lane; the whole state lives in VGPRs.
+#pragma once
+#include "st_physics.h"
+
+namespace st {
+
+// Context-resident scene: sky, LUTs, disc, gas grid, stars.
+struct DeviceScene {
+    const float4* sky;       // [sky_h][sky_w] RGB(A)
+    int sky_w, sky_h;
+    const float4* lut;       // disc blackbody LUT
+    int lut_n;
+    float lut_tmin, lut_tmax;
+    int disc_on;
+    float disc_inner, disc_outer, disc_falloff;
+    float T0, Tr0;
+    int T_hold;
+    const float* vol;        // [nz][nphi][nr]
+    int nr, nphi, nz;
+    float vol_params[9];
+    const float4* star_lut;
+    int star_lut_n;
+    float star_params[16];
+};
+
+// Per-launch parameters.
+struct TraceParams {
+    float M, a, r_escape;
+    int relativistic;
+    float h_coef, h_cap, rkill2;
+    int nmax;
+    float dt;
+    float ju, jv, jw;
+    int jseed;
+    float weight;
+    int order;
+    float h_floor;
+    int capture_mode;
+    float capture_factor;
+    int kerr_shrink, esc_interp, bardeen;
+    float promote_band, promote_horizon;
+    int assign;              // 1: out = sample (batched frames); 0: out += weight * sample
+    float* out;
+    float* rec;              // ST_REC_N floats per pixel, or null
+    uint2* queue;            // promoted rays (frame, pixel)
+    unsigned* qcount;
+    unsigned long long* counters;  // [0] steps fp32 [1] steps fp64 [2] unused [3] rays
+};
+
+// One frame of a launch (a single render or one element of a batch).  Passed
+// by value as a kernel argument for single renders (the camera block is then
+// read with scalar loads from the argument segment) and stored in a device
+// array for batches.
+struct FrameRef {
+    float c32[49];           // cam_params[49]
+    double c64[49];          // FP64 copy of the same block
+    int has64;               // c64 valid
+    int width, height;
+    float* out;              // frame output base
+    float* rec;              // frame record base (may be null)
+};
+
+template <typename T> struct RayOut {
+    T rgb[3];
+    T dir[3];
+    T ham, r_end, rk_min;
+    int fate, steps, flags;
+    bool routed;
+};
+
+enum { REC_N = 12 };
+
+template <typename T>
+__device__ __forceinline__ T cam_at(const FrameRef& F, int k) {
+    if constexpr (sizeof(T) == 8) {
+        return F.has64 ? F.c64[k] : (T)F.c32[k];
+    } else {
+        return F.c32[k];
+    }
+}
+
+// Bilinear sky fetch, sample_background_mtl (src/metal.jl:523-561), no phi
+// wrap at the seam.
+template <typename T>
+__device__ __forceinline__ void sample_sky(const DeviceScene& D, T theta, T phi, T col[3]) {
+    using A = Ar<T>;
+    const T two_pi = T(6.283185307179586);
+    const T v = clampv(theta / T(3.141592653589793), T(0), T(1));
+    const T u = clampv(jmod(phi, two_pi) / two_pi, T(0), T(1));
+    const int W = D.sky_w, H = D.sky_h;
+    const T xf = u * T(W - 1), yf = v * T(H - 1);
+    int x0 = (int)A::floor(xf);
+    x0 = x0 < 0 ? 0 : (x0 > W - 1 ? W - 1 : x0);
+    const int x1 = x0 + 1 < W - 1 ? x0 + 1 : W - 1;
+    int y0 = (int)A::floor(yf);
+    y0 = y0 < 0 ? 0 : (y0 > H - 1 ? H - 1 : y0);
+    const int y1 = y0 + 1 < H - 1 ? y0 + 1 : H - 1;
+    const T fx = xf - T(x0), fy = yf - T(y0);
+    const float4 c00 = D.sky[y0 * W + x0], c10 = D.sky[y0 * W + x1];
+    const float4 c01 = D.sky[y1 * W + x0], c11 = D.sky[y1 * W + x1];
+    const T w00 = (T(1) - fx) * (T(1) - fy), w10 = fx * (T(1) - fy), w01 = (T(1) - fx) * fy, w11 = fx * fy;
+    col[0] = T(c00.x) * w00 + T(c10.x) * w10 + T(c01.x) * w01 + T(c11.x) * w11;
+    col[1] = T(c00.y) * w00 + T(c10.y) * w10 + T(c01.y) * w01 + T(c11.y) * w11;
+    col[2] = T(c00.z) * w00 + T(c10.z) * w10 + T(c01.z) * w01 + T(c11.z) * w11;
+}
+
+// Trilinear gas density, sample_volume_mtl (src/metal.jl:569-610).
+template <typename T>
+__device__ __forceinline__ T sample_volume(const DeviceScene& D, T s, T phi, T z) {
+    using A = Ar<T>;
+    const T zmax = T(D.vol_params[2]);
+    if (s <= T(0) || A::fabs(z) >= zmax) return T(0);
+    const T ls = A::log(s);
+    const T ls_in = T(D.vol_params[0]), ls_out = T(D.vol_params[1]);
+    if (ls <= ls_in || ls >= ls_out) return T(0);
+    const int nr = D.nr, nphi = D.nphi, nz = D.nz;
+    const T two_pi = T(6.283185307179586);
+    const T fr = (ls - ls_in) / (ls_out - ls_in) * T(nr - 1);
+    const T fp = jmod(phi, two_pi) / two_pi * T(nphi);
+    const T fz = (z + zmax) / (T(2) * zmax) * T(nz - 1);
+    int i0 = (int)A::floor(fr);
+    i0 = i0 < 0 ? 0 : (i0 > nr - 2 ? nr - 2 : i0);
+    const T tr = fr - T(i0);
+    const int j0 = (int)A::floor(fp);
+    const T tp = fp - T(j0);
+    int k0 = (int)A::floor(fz);
+    k0 = k0 < 0 ? 0 : (k0 > nz - 2 ? nz - 2 : k0);
+    const T tz = fz - T(k0);
+    int j0a = j0 % nphi;
+    if (j0a < 0) j0a += nphi;
+    int j1a = (j0 + 1) % nphi;
+    if (j1a < 0) j1a += nphi;
+    const float* v = D.vol;
+    auto d = [&](int ii, int jj, int kk) { return T(v[((size_t)kk * nphi + jj) * nr + ii]); };
+    const T c00 = d(i0, j0a, k0) + tr * (d(i0 + 1, j0a, k0) - d(i0, j0a, k0));
+    const T c10 = d(i0, j1a, k0) + tr * (d(i0 + 1, j1a, k0) - d(i0, j1a, k0));
+    const T c01 = d(i0, j0a, k0 + 1) + tr * (d(i0 + 1, j0a, k0 + 1) - d(i0, j0a, k0 + 1));
+    const T c11 = d(i0, j1a, k0 + 1) + tr * (d(i0 + 1, j1a, k0 + 1) - d(i0, j1a, k0 + 1));
+    const T c0 = c00 + tp * (c10 - c00), c1 = c01 + tp * (c11 - c01);
+    return c0 + tz * (c1 - c0);
+}
+
+// Procedural point stars, starfield_mtl (src/metal.jl:652-724).
+template <typename T>
+__device__ void starfield(const DeviceScene& D, T dx, T dy, T dz, T out[3]) {
+    using A = Ar<T>;
+    const float* sp = D.star_params;
+    const T N = T(sp[1]), fill = T(sp[2]), sig = T(sp[3]), flux0 = T(sp[4]);
+    const T gnx = T(sp[5]), gny = T(sp[6]), gnz = T(sp[7]), gconc = T(sp[8]);
+    const T tmin = T(sp[9]), tspan = T(sp[10]), lut_tmin = T(sp[11]), lut_tmax = T(sp[12]), lut_size = T(sp[13]);
+    const int seed = (int)sp[14];
+    const T two_pi = T(6.283185307179586);
+    const T u = A::atan2(dy, dx) / two_pi + T(0.5);
+    const T v = (dz + T(1)) * T(0.5);
+    const int Ni = (int)sp[1];
+    const int i0 = (int)A::floor(u * N), j0 = (int)A::floor(v * N);
+    const T inv2s2 = T(1) / (T(2) * sig * sig), cut = T(25) * sig * sig;
+    out[0] = out[1] = out[2] = T(0);
+    for (int dj = -1; dj <= 1; ++dj) {
+        const int jj = j0 + dj;
+        if (jj < 0 || jj >= Ni) continue;
+        for (int di = -1; di <= 1; ++di) {
+            int ii = (i0 + di) % Ni;
+            if (ii < 0) ii += Ni;
+            const T su = T(sim_hash(ii, jj, seed + 1)), sv = T(sim_hash(ii, jj, seed + 2));
+            const T phs = ((T(ii) + su) / N - T(0.5)) * two_pi;
+            const T sz = T(2) * (T(jj) + sv) / N - T(1);
+            const T sr = A::psqrt(mx(T(1) - sz * sz, T(0)));
+            const T sx = sr * A::cos(phs), sy = sr * A::sin(phs);
+            const T ex = dx - sx, ey = dy - sy, ez = dz - sz;
+            const T d2 = ex * ex + ey * ey + ez * ez;
+            if (d2 > cut) continue;
+            T occ = fill;
+            if (gconc > T(0)) occ *= A::exp(-A::fabs(sx * gnx + sy * gny + sz * gnz) * gconc);
+            if (!(T(sim_hash(ii, jj, seed)) < occ)) continue;
+            const T xi = mx(T(sim_hash(ii, jj, seed + 3)), T(1e-4));
+            const T flux = flux0 * A::exp(T(-0.6666667) * A::log(xi));
+            const T w = flux * A::exp(-d2 * inv2s2);
+            const T Tt = tmin + tspan * T(sim_hash(ii, jj, seed + 4));
+            const T frac = (clampv(Tt, lut_tmin, lut_tmax) - lut_tmin) / mx(lut_tmax - lut_tmin, T(1e-6));
+            int li = (int)(frac * (lut_size - T(1)) + T(0.5));
+            li = li < 0 ? 0 : (li > D.star_lut_n - 1 ? D.star_lut_n - 1 : li);
+            const float4 c = D.star_lut[li];
+            out[0] += w * T(c.x);
+            out[1] += w * T(c.y);
+            out[2] += w * T(c.z);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void lut_color(const DeviceScene& D, T T_obs, T col[3]) {
+    const T tmin = T(D.lut_tmin), tmax = T(D.lut_tmax);
+    const T frac = (clampv(T_obs, tmin, tmax) - tmin) / (tmax - tmin);
+    int li = (int)(frac * (T(D.lut_n) - T(1)) + T(0.5));
+    li = li < 0 ? 0 : (li > D.lut_n - 1 ? D.lut_n - 1 : li);
+    const float4 c = D.lut[li];
+    col[0] = T(c.x); col[1] = T(c.y); col[2] = T(c.z);
+}
+
+// Disc/gas emission temperature: T0 (s_T / r0)^-3/4 with s_T = max(s, r_isco)
+// when the ISCO hold is on (src/metal.jl:1528-1531, 1469-1471).
+template <typename T>
+__device__ __forceinline__ T temp_law(const DeviceScene& D, T s, T r_isco) {
+    using A = Ar<T>;
+    const T sT = D.T_hold ? mx(s, r_isco) : s;
+    return T(D.T0) * A::exp(T(-0.75) * A::log(mx(sT / T(D.Tr0), T(1e-6))));
+}
+
+template <typename T>
+struct Kin {  // disc kinematics shared by the shading blocks
+    T r_isco, E_isco, L_isco, scam, pt, M, a;
+};
+
+// Thin-plane crossing shading, src/metal.jl:1524-1569.
+template <typename T>
+__device__ __noinline__ void shade_disc(const DeviceScene& D, const Kin<T>& K, T xh, T yh, T pxh, T pyh,
+                                        T& alpha, T acc[3]) {
+    using A = Ar<T>;
+    const T s = A::psqrt(xh * xh + yh * yh);
+    const T inner = T(D.disc_inner), outer = T(D.disc_outer);
+    if (!(inner < s && s < outer)) return;
+    const T M = K.M;
+    const T R = s / (T(2) * M);
+    const T T_emit = temp_law(D, s, K.r_isco);
+    T plunge = T(1);
+    if (D.T_hold && s < K.r_isco) {
+        const T q = s / K.r_isco;
+        plunge = q * q * q;
+    }
+    const T opz = emitter_opz(xh, yh, pxh, pyh, -K.pt, M, K.a, K.r_isco, K.E_isco, K.L_isco);
+    const T T_obs = T_emit * K.scam / opz;
+    const T inten = T(100) / (A::exp(T(29622.4) / mx(T_obs, T(1))) - T(1));
+    T col[3];
+    lut_color(D, T_obs, col);
+    const T R_in = inner / (T(2) * M), R_out = outer / (T(2) * M);
+    const T iscotaper = clampv((R * R - R_in * R_in) * T(0.3), T(0), T(1));
+    const T outertaper = clampv(T_emit / T(1000), T(0), T(1));
+    const T density = clampv((R_out - R) / (R_out - R_in), T(0), T(1));
+    const T falloff = T(D.disc_falloff);
+    const T dpow = density <= T(1e-6) ? (falloff > T(0) ? T(0) : T(1)) : A::exp(falloff * A::log(density));
+    const T opacity = iscotaper * outertaper * dpow * plunge;
+    const T w = alpha * opacity * inten;
+    acc[0] += w * col[0];
+    acc[1] += w * col[1];
+    acc[2] += w * col[2];
+    alpha *= (T(1) - opacity);
+}
+
+// Arc-length gas march over one accepted step, src/metal.jl:1440-1513.
+template <typename T>
+__device__ __noinline__ void march_gas(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
+                                       T seg, T ds_gas, T& march_acc, T& alpha, T acc[3]) {
+    using A = Ar<T>;
+    // a non-finite or absurd step length must not spin the sample loop
+    if (!(seg >= T(0) && seg < T(1e6))) return;
+    T t_s = ds_gas - march_acc;
+    if (t_s > seg) {
+        march_acc += seg;
+        return;
+    }
+    const T zmax = T(D.vol_params[2]);
+    int guard = 0;
+    while (t_s <= seg && alpha > T(0.003) && ++guard <= 65536) {
+        const T fs = t_s / seg;
+        const T sx = sp.x + fs * (sn.x - sp.x);
+        const T sy = sp.y + fs * (sn.y - sp.y);
+        const T sz = sp.z + fs * (sn.z - sp.z);
+        if (A::fabs(sz) < zmax) {
+            const T s_cyl = A::psqrt(sx * sx + sy * sy);
+            if (s_cyl > T(1e-6)) {
+                T rho = sample_volume(D, s_cyl, A::atan2(sy, sx), sz);
+                if (rho > T(1e-4)) {
+                    const T T_emit = temp_law(D, s_cyl, K.r_isco);
+                    if (s_cyl < K.r_isco) {
+                        const T q = s_cyl / K.r_isco;
+                        rho *= q * q * q;
+                    }
+                    const T psx = sp.px + fs * (sn.px - sp.px);
+                    const T psy = sp.py + fs * (sn.py - sp.py);
+                    const T opz = emitter_opz(sx, sy, psx, psy, -K.pt, K.M, K.a, K.r_isco, K.E_isco, K.L_isco);
+                    const T T_obs = T_emit * K.scam / opz;
+                    const T inten = T(100) / (A::exp(T(29622.4) / mx(T_obs, T(1))) - T(1));
+                    T col[3];
+                    lut_color(D, T_obs, col);
+                    const T tau = T(D.vol_params[7]) * rho * ds_gas;
+                    const T aa = T(1) - A::exp(-tau);
+                    const T w = alpha * aa * inten * T(D.vol_params[6]);
+                    acc[0] += w * col[0];
+                    acc[1] += w * col[1];
+                    acc[2] += w * col[2];
+                    alpha *= (T(1) - aa);
+                }
+            }
+        }
+        t_s += ds_gas;
+    }
+    march_acc = seg - (t_s - ds_gas);
+}
+
+// Launch-time radial-potential analysis (src/metal.jl:1008-1094), for a = 0
+// as well.  Returns captured (no turning point before the horizon), graze,
+// and the outermost critical point rc of R(r) in (r_plus, r_cam) with R(rc)
+// and R''(rc): the photon-shell radius this ray winds at.
+template <typename T, bool KERR>
+__device__ __forceinline__ void bardeen(const S6<T>& s, T pt, T M, T a, bool& captured, bool& graze, bool& ingoing,
+                                        T& rc_out, T& Rc_out, T& Rpp_out) {
+    using A = Ar<T>;
+    captured = false; graze = false; ingoing = false;
+    rc_out = T(-1); Rc_out = T(0); Rpp_out = T(1);
+    const T E0 = -pt;
+    if (!(A::fabs(E0) > T(1e-6))) return;
+    const T a2 = a * a;
+    const T aw = s.x * s.x + s.y * s.y + s.z * s.z - a2;
+    const T rk2 = T(0.5) * (aw + A::psqrt(aw * aw + T(4) * a2 * s.z * s.z));
+    const T rk = A::psqrt(mx(rk2, T(1e-12)));
+    const T cth = s.z / rk;
+    const T s2 = mx(T(1) - cth * cth, T(1e-8));
+    const T sth = A::psqrt(s2);
+    const T Lz0 = s.x * s.py - s.y * s.px;
+    const T pth = (cth / sth) * (s.x * s.px + s.y * s.py) - rk * sth * s.pz;
+    const T Q0 = pth * pth + cth * cth * (Lz0 * Lz0 / s2 - a2 * E0 * E0);
+    const T lam = Lz0 / E0, eta = Q0 / (E0 * E0);
+    const T c2 = a2 - lam * lam - eta;
+    const T c1 = T(2) * M * (eta + (lam - a) * (lam - a));
+    const T c0 = -a2 * eta;
+    const T rplus = M + A::psqrt(mx(M * M - a2, T(0)));
+    T rk2d;
+    const S6<T> v = rhs<T, KERR>(s, pt, M, a, rk2d);
+    const T rdot = rk2 * (s.x * v.x + s.y * v.y) + (rk2 + a2) * s.z * v.z;
+    if (!(rdot < T(0))) return;
+    ingoing = true;
+    const T pb = T(0.5) * c2, qb = T(0.25) * c1;
+    const T Db = T(0.25) * qb * qb + pb * pb * pb / T(27);
+    bool cap = true;
+    T best = T(-1), bestR = T(0);
+    auto cbrt_s = [](T x) {
+        const T m = A::exp(A::log(mx(A::fabs(x), T(1e-30))) / T(3));
+        return x < T(0) ? -m : (x > T(0) ? m : T(0));
+    };
+    if (Db >= T(0)) {
+        const T sD = A::psqrt(Db);
+        const T rc = cbrt_s(T(-0.5) * qb + sD) + cbrt_s(T(-0.5) * qb - sD);
+        if (rc > rplus && rc < rk) {
+            const T Rv = ((rc * rc + c2) * rc + c1) * rc + c0;
+            cap = Rv > T(0);
+            if (A::fabs(Rv) < T(0.08) * mx(rc * rc * rc * rc, T(1))) graze = true;
+            best = rc; bestR = Rv;
+        }
+    } else {
+        const T mb = T(2) * A::psqrt(-pb / T(3));
+        const T ac = clampv(T(3) * qb / (pb * mb), T(-1), T(1));
+        const T thb = A::acos(ac) / T(3);
+        for (int kk = 0; kk < 3; ++kk) {
+            const T rc = mb * A::cos(thb - T(2.0943951023931953) * T(kk));
+            if (rc > rplus && rc < rk) {
+                const T Rv = ((rc * rc + c2) * rc + c1) * rc + c0;
+                if (Rv <= T(0)) cap = false;
+                if (A::fabs(Rv) < T(0.08) * mx(rc * rc * rc * rc, T(1))) graze = true;
+                if (rc > best) { best = rc; bestR = Rv; }
+            }
+        }
+    }
+    captured = cap;
+    rc_out = best;
+    Rc_out = bestR;
+    Rpp_out = T(12) * best * best + T(2) * c2;
+}
+
+// Trace one ray of frame F through pixel (i, j).  ROUTE: FP32 pass of the
+// mixed policy — rays that graze their photon-shell radius within
+// promote_band, that the launch analysis finds captured, or that come within
+// promote_horizon * r_plus of the horizon are handed to the FP64 queue
+// (o.routed) instead of being finished here.
+template <typename T, bool KERR, bool VOL, bool ROUTE>
+__device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParams& P, const FrameRef& F, int i, int j,
+                                          RayOut<T>& o) {
+    using A = Ar<T>;
+    const int I = i + 1, J = j + 1;   // the reference hashes 1-based pixel indices
+    const T M = T(P.M), a = KERR ? T(P.a) : T(0);
+    o.routed = false;
+    o.flags = (sizeof(T) == 8) ? 1 : 0;
+
+    // ---- pose (optional per-pixel shutter, src/metal.jl:833-848) ----
+    T cp[20];
+#pragma unroll
+    for (int k = 0; k < 20; ++k) cp[k] = cam_at<T>(F, k);
+    if (F.c32[28] > 0.0f) {
+        const T tau = T(sim_hash(I, J, (int)F.c32[28]));
+#pragma unroll
+        for (int k = 0; k < 20; ++k)
+            if (k != 3) cp[k] += tau * (cam_at<T>(F, 29 + k) - cp[k]);
+    }
+    // ---- sensor coordinate + jitter (src/metal.jl:855-866) ----
+    T ju = T(P.ju), jv = T(P.jv);
+    if (P.jw > 0.0f) {
+        ju += T(sim_hash(I, J, P.jseed)) * T(P.jw);
+        jv += T(sim_hash(I, J, P.jseed + 271)) * T(P.jw);
+    }
+    const T half_h = T(F.height) / T(2);
+    const T u = (T(I) - T(1) + ju - T(F.width) / T(2)) / half_h;
+    const T v = (T(J) - T(1) + jv - T(F.height) / T(2)) / half_h;
+    // ---- thin-lens aperture sample (src/metal.jl:881-903) ----
+    T offr = T(0), offu = T(0);
+    if (F.c32[25] > 0.0f) {
+        const int sd = (int)F.c32[27];
+        const T u01 = cam_at<T>(F, 22) + T(sim_hash(I, J, sd)) * cam_at<T>(F, 25);
+        const T v01 = cam_at<T>(F, 23) + T(sim_hash(I, J, sd + 7919)) * cam_at<T>(F, 25);
+        const T ox = T(2) * u01 - T(1), oy = T(2) * v01 - T(1);
+        if (ox != T(0) || oy != T(0)) {
+            T rr, th;
+            if (A::fabs(ox) > A::fabs(oy)) { rr = ox; th = T(0.785398) * (oy / ox); }
+            else { rr = oy; th = T(1.570796) - T(0.785398) * (ox / oy); }
+            offr = cam_at<T>(F, 26) * rr * A::cos(th);
+            offu = cam_at<T>(F, 26) * rr * A::sin(th);
+        }
+    }
+    // ---- projection (src/metal.jl:907-949) ----
+    T cr, cu, cf;
+    const float proj = F.c32[20];
+    if (proj > 1.5f) {
+        const T phe = u * T(1.5707963), the = (v + T(1)) * T(1.5707963);
+        const T sth = A::sin(the);
+        cf = A::cos(the); cr = sth * A::cos(phe); cu = sth * A::sin(phe);
+    } else if (proj > 0.5f) {
+        const T rho = A::psqrt(u * u + v * v);
+        const T tp = rho * cam_at<T>(F, 21);
+        const T sth = A::sin(tp);
+        const T inv_rho = rho > T(1e-8) ? T(1) / rho : T(0);
+        cr = sth * u * inv_rho; cu = sth * v * inv_rho; cf = A::cos(tp);
+    } else {
+        T dx = u * cp[3], dy = v * cp[3];
+        T nu = A::psqrt(dx * dx + dy * dy + T(1));
+        if (offr != T(0) || offu != T(0)) {
+            const T k = nu / cam_at<T>(F, 24);
+            dx -= offr * k; dy -= offu * k;
+            nu = A::psqrt(dx * dx + dy * dy + T(1));
+        }
+        const T inu = T(1) / nu;
+        cr = dx * inu; cu = dy * inu; cf = inu;
+    }
+    // ---- KS photon init (src/metal.jl:951-995) ----
+    S6<T> s;
+    s.x = cp[0] + offr * cp[9] + offu * cp[13];
+    s.y = cp[1] + offr * cp[10] + offu * cp[14];
+    s.z = cp[2] + offr * cp[11] + offu * cp[15];
+    const T qt = cf * cp[4] + cr * cp[8] + cu * cp[12] - cp[16];
+    const T qx = cf * cp[5] + cr * cp[9] + cu * cp[13] - cp[17];
+    const T qy = cf * cp[6] + cr * cp[10] + cu * cp[14] - cp[18];
+    const T qz = cf * cp[7] + cr * cp[11] + cu * cp[15] - cp[19];
+    T pt;
+    const T r_cam = A::psqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+    if constexpr (KERR) {
+        const T a2 = a * a;
+        const T wl = r_cam * r_cam - a2;
+        const T rk2 = T(0.5) * (wl + A::psqrt(wl * wl + T(4) * a2 * s.z * s.z));
+        const T rk = A::psqrt(mx(rk2, T(1e-12)));
+        const T iRA = T(1) / (rk2 + a2);
+        const T lx = (rk * s.x + a * s.y) * iRA, ly = (rk * s.y - a * s.x) * iRA, lz = s.z / rk;
+        const T fl = T(2) * M * rk2 * rk / (rk2 * rk2 + a2 * s.z * s.z);
+        const T lq = qt + lx * qx + ly * qy + lz * qz;
+        const T flq = fl * lq;
+        pt = -qt + flq;
+        s.px = qx + flq * lx; s.py = qy + flq * ly; s.pz = qz + flq * lz;
+    } else {
+        const T f = T(2) * M / r_cam;
+        const T lq = qt + (s.x * qx + s.y * qy + s.z * qz) / r_cam;
+        pt = -qt + f * lq;
+        const T flr = f * lq / r_cam;
+        s.px = qx + flr * s.x; s.py = qy + flr * s.y; s.pz = qz + flr * s.z;
+    }
+    Kin<T> K;
+    K.pt = pt; K.M = M; K.a = a;
+    K.scam = P.relativistic ? T(1) / clampv(A::fabs(pt), T(0.05), T(20)) : T(1);
+    isco(M, a, K.r_isco, K.E_isco, K.L_isco);
+
+    // ---- launch-time radial potential (src/metal.jl:1008-1094) ----
+    bool captured0 = false;
+    const T rplus = M + A::psqrt(mx(M * M - a * a, T(0)));
+    const bool need_bardeen = (KERR && P.bardeen && P.a != 0.0f) || ROUTE;
+    if (need_bardeen) {
+        bool cap, graze, ing;
+        T rc, Rc, Rpp;
+        bardeen<T, KERR>(s, pt, M, a, cap, graze, ing, rc, Rc, Rpp);
+        if (KERR && P.bardeen && P.a != 0.0f) captured0 = cap && ing;
+        if (cap && ing) o.flags |= 2;
+        if (graze) o.flags |= 4;
+        if constexpr (ROUTE) {
+            const T band = T(P.promote_band);
+            bool promote = cap && ing;
+            if (ing && !cap && rc > T(0) && Rpp > T(0)) promote = promote || (T(-2) * Rc < band * band * Rpp);
+            if (promote) {
+                o.routed = true;
+                return;
+            }
+        }
+    }
+
+    // ---- integration (src/metal.jl:1201-1572) ----
+    const bool disc_plane = !VOL && D.disc_on && D.disc_inner > 0.0f && D.disc_outer > D.disc_inner;
+    const T vol_zmax = T(D.vol_params[2]);
+    const T vol_s_out = VOL ? A::exp(T(D.vol_params[1])) : T(0);
+    const T vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
+    int mstep = (int)D.vol_params[8];
+    mstep = mstep < 1 ? 1 : (mstep > 16 ? 16 : mstep);
+    const T ds_gas = T(mstep) * T(P.dt);
+    const T r_escape = T(P.r_escape), dt = T(P.dt), h_coef = T(P.h_coef), h_floor = T(P.h_floor);
+    const T h_cap = T(P.h_cap), rkill2 = T(P.rkill2);
+    const T rcap = T(P.capture_factor) * rplus;
+    const T rcap2 = rcap * rcap;
+    const T rprom = T(P.promote_horizon) * rplus;
+    const T rprom2 = rprom * rprom;
+    const T invM = T(1) / M;
+    const T r_floor = r_cam > T(2.05) * M ? T(2) * M : T(0.3) * M;
+    T r_prev = T(-1);
+    T alpha = T(1);
+    T acc[3] = {T(0), T(0), T(0)};
+    T march_acc = T(0);
+    T rk_min = T(1e30);
+    // previous step start radius and velocity (escape-direction interpolation)
+    T r_start = T(-1), v0x = T(0), v0y = T(0), v0z = T(0);
+    bool hit = false, nonfinite = false, escaped = false, routed = false;
+    int steps = 0;
+    const int order = P.order;
+    for (int it = 0; it < P.nmax; ++it) {
+        T rk2;
+        const S6<T> k1 = rhs<T, KERR>(s, pt, M, a, rk2);
+        const T r2 = s.x * s.x + s.y * s.y + s.z * s.z;
+        const T r = (sizeof(T) == 4) ? r2 * A::rsq(r2) : A::sqrt(r2);
+        if (P.rec) rk_min = mn(rk_min, rk2);
+        if (P.capture_mode == 1) {
+            if (rk2 < rcap2) { hit = true; break; }
+        } else if (KERR) {
+            if (rk2 < rkill2) { hit = true; break; }
+        } else if (r < T(3.2) * M) {
+            if (r < r_floor || (r < T(2.95) * M && r_prev > T(0) && r < r_prev - T(1e-4) * M)) { hit = true; break; }
+            r_prev = r;
+        }
+        if constexpr (ROUTE) {
+            if (rk2 < rprom2) { routed = true; break; }
+        }
+        if (r > r_escape) { escaped = true; break; }
+        const T hcap = (VOL && r2 < vol_rb2) ? T(2) : h_cap;
+        T h = dt * mn(mx(h_coef * r * invM, h_floor), hcap);
+        if (KERR && P.kerr_shrink) {
+            const T fr = clampv((rk2 - rkill2) / mx(rkill2, T(1e-6)), T(0), T(1));
+            h *= T(0.12) + T(0.88) * fr;
+        }
+        const S6<T> sp = s;
+        // Capture mode 1 also captures a ray whose RK stage point falls inside
+        // the capture radius: below the prograde photon orbit no turning point
+        // exists, and a backward ray nearing r_plus has a covariant momentum
+        // that diverges at finite affine parameter, so a step straddling the
+        // radius must not be completed.
+        T rk2d, stage_min;
+        if (order == 4) {
+            const T hh = T(0.5) * h;
+            S6<T> accum = k1;
+            S6<T> k = rhs<T, KERR>(axpy(s, hh, k1), pt, M, a, rk2d);
+            stage_min = rk2d;
+            accum = axpy(accum, T(2), k);
+            k = rhs<T, KERR>(axpy(s, hh, k), pt, M, a, rk2d);
+            stage_min = mn(stage_min, rk2d);
+            accum = axpy(accum, T(2), k);
+            k = rhs<T, KERR>(axpy(s, h, k), pt, M, a, rk2d);
+            stage_min = mn(stage_min, rk2d);
+            if (P.capture_mode == 1 && stage_min < rcap2) { hit = true; break; }
+            accum = axpy(accum, T(1), k);
+            s = axpy(s, h * T(1.0 / 6.0), accum);
+        } else if (order == 2) {
+            const S6<T> k2 = rhs<T, KERR>(axpy(s, T(0.5) * h, k1), pt, M, a, rk2d);
+            if (P.capture_mode == 1 && rk2d < rcap2) { hit = true; break; }
+            s = axpy(s, h, k2);
+        } else {
+            s = axpy(s, h, k1);
+        }
+        ++steps;
+        r_start = r; v0x = k1.x; v0y = k1.y; v0z = k1.z;
+        if (!(s.x == s.x) || !(s.z == s.z) || !(s.px == s.px) ||
+            !(A::fabs(s.x) + A::fabs(s.y) + A::fabs(s.z) < T(1e30))) {
+            hit = true; nonfinite = true; break;
+        }
+        if constexpr (VOL) {
+            const T rn2 = s.x * s.x + s.y * s.y + s.z * s.z;
+            if (alpha > T(0.003) && mn(r2, rn2) < vol_rb2) {
+                const T vlen = mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
+                march_gas(D, K, sp, s, h * vlen, ds_gas, march_acc, alpha, acc);
+                if (alpha < T(0.003)) break;
+            }
+        }
+        if (disc_plane && alpha > T(0.001) && sp.z * s.z < T(0)) {
+            const T cfr = sp.z / (sp.z - s.z);
+            shade_disc(D, K, sp.x + cfr * (s.x - sp.x), sp.y + cfr * (s.y - sp.y), sp.px + cfr * (s.px - sp.px),
+                       sp.py + cfr * (s.py - sp.py), alpha, acc);
+        }
+    }
+    o.steps = steps;
+    if constexpr (ROUTE) {
+        if (routed) {
+            o.routed = true;
+            return;
+        }
+    }
+    // ---- fate + sky (src/metal.jl:1629-1704) ----
+    const T rf = mx(A::psqrt(s.x * s.x + s.y * s.y + s.z * s.z), T(1e-6));
+    o.r_end = rf;
+    o.rk_min = A::psqrt(mx(rk_min, T(0)));
+    o.ham = nonfinite ? T(NAN) : hamiltonian<T, KERR>(s, pt, M, a) / (pt * pt);
+    if (hit || captured0 || rf < T(4) * M) {
+        o.fate = nonfinite ? 2 : 1;
+        o.rgb[0] = acc[0]; o.rgb[1] = acc[1]; o.rgb[2] = acc[2];
+        o.dir[0] = o.dir[1] = o.dir[2] = T(0);
+        return;
+    }
+    T rk2d;
+    const S6<T> vf = rhs<T, KERR>(s, pt, M, a, rk2d);
+    T vl = mx(A::psqrt(vf.x * vf.x + vf.y * vf.y + vf.z * vf.z), T(1e-20));
+    T dx = vf.x / vl, dy = vf.y / vl, dz = vf.z / vl;
+    if (P.esc_interp && escaped && r_start > T(0) && r_start < r_escape && rf > r_escape) {
+        const T t = (r_escape - r_start) / (rf - r_start);
+        const T v0l = mx(A::psqrt(v0x * v0x + v0y * v0y + v0z * v0z), T(1e-20));
+        const T ax = v0x / v0l, ay = v0y / v0l, az = v0z / v0l;
+        const T ix = ax + t * (dx - ax), iy = ay + t * (dy - ay), iz = az + t * (dz - az);
+        const T il = A::psqrt(ix * ix + iy * iy + iz * iz);
+        dx = ix / il; dy = iy / il; dz = iz / il;
+    }
+    o.dir[0] = dx; o.dir[1] = dy; o.dir[2] = dz;
+    const T th = A::acos(clampv(dz, T(-1), T(1)));
+    const T ph = A::atan2(dy, dx);
+    T col[3];
+    sample_sky(D, th, ph, col);
+    if (D.star_params[0] > 0.0f) {
+        const T tw = T(D.star_params[15]);
+        T stc[3];
+        starfield(D, dx, dy, dz, stc);
+        const T sw = T(D.star_params[0]);
+        col[0] = col[0] * tw + sw * stc[0];
+        col[1] = col[1] * tw + sw * stc[1];
+        col[2] = col[2] * tw + sw * stc[2];
+    }
+    if (K.scam != T(1)) {
+        col[0] *= T(57.4) / (A::exp(T(4.067) / K.scam) - T(1));
+        col[1] *= T(90.2) / (A::exp(T(4.513) / K.scam) - T(1));
+        col[2] *= T(206.5) / (A::exp(T(5.335) / K.scam) - T(1));
+    }
+    o.fate = 0;
+    o.rgb[0] = acc[0] + alpha * col[0];
+    o.rgb[1] = acc[1] + alpha * col[1];
+    o.rgb[2] = acc[2] + alpha * col[2];
+}
+
+// Write one finished ray: accumulate (or assign) the sample, and the record.
+template <typename T>
+__device__ __forceinline__ void store_ray(const TraceParams& P, const FrameRef& F, int i, int j, const RayOut<T>& o) {
+    const size_t px = (size_t)j * F.width + i;
+    float* out = F.out + px * 3;
+    if (P.assign) {
+        out[0] = (float)o.rgb[0];
+        out[1] = (float)o.rgb[1];
+        out[2] = (float)o.rgb[2];
+    } else {
+        const T w = T(P.weight);
+        out[0] += (float)(w * o.rgb[0]);
+        out[1] += (float)(w * o.rgb[1]);
+        out[2] += (float)(w * o.rgb[2]);
+    }
+    if (F.rec) {
+        float* r = F.rec + px * REC_N;
+        r[0] = (float)o.fate;
+        r[1] = (float)o.steps;
+        r[2] = (float)o.dir[0]; r[3] = (float)o.dir[1]; r[4] = (float)o.dir[2];
+        r[5] = (float)o.rgb[0]; r[6] = (float)o.rgb[1]; r[7] = (float)o.rgb[2];
+        r[8] = (float)o.ham;
+        r[9] = (float)o.r_end;
+        r[10] = (float)o.flags;
+        r[11] = (float)o.rk_min;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace st

@@ -1,0 +1,327 @@
+"""GPU parity: libspacetime_hip (through the C ABI) against the FP64 oracle on
+the same seeded inputs.
+
+Bars (per-pixel, outside a one-pixel band at the shadow and photon-ring
+edge, where fate or direction changes within a pixel):
+  fate:       identical
+  direction:  <= 1e-3 rad
+  radiance:   <= 1e-3 relative (a share of disc pixels sits on a LUT-bin
+              or disc-edge boundary; the fraction within 1e-3 is asserted)
+  H drift:    reported (|H| / p_t^2 at termination)
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from spacetime_amd import _lib as L
+from spacetime_amd import api, scenes
+from util import angle_between, oracle_scene
+
+pytestmark = pytest.mark.gpu
+
+DIR_TOL = 1e-3
+RAD_TOL = 1e-3
+
+
+def gpu_trace(sc: scenes.Scene, precision=L.ST_PREC_FP32, sky=None, records=True, rule=None, ctx=None,
+              **args_over):
+    own = ctx is None
+    if own:
+        ctx = api.HipPreviewContext(scenes.sky() if sky is None else sky, sc.width, sc.height, disc=sc.disc)
+    rule = sc.rule if rule is None else rule
+    if records:
+        ctx.enable_records(True)
+    c32, c64 = sc.cam_params()
+    sp = api.spacetime_params(sc.spacetime, sc.camera, rule, sc.relativistic)
+    args = api.trace_args(sc.width, sc.height, rule, precision=precision)
+    for k, v in args_over.items():
+        setattr(args, k, v)
+    ctx.clear()
+    ctx.trace(c32, c64, sp, args)
+    img = ctx.download()
+    rec = ctx.records() if records else None
+    stats = ctx.stats()
+    if own:
+        ctx.close()
+    return img, rec, stats
+
+
+def edge_band(fate: np.ndarray, dirs: np.ndarray | None = None, spread=0.05) -> np.ndarray:
+    """Pixels whose 3x3 neighbourhood holds a different fate, or (escaped)
+    whose direction turns by more than `spread` rad to a neighbour."""
+    H, W = fate.shape
+    band = np.zeros_like(fate, dtype=bool)
+    pad = np.pad(fate, 1, mode="edge")
+    for dj in (-1, 0, 1):
+        for di in (-1, 0, 1):
+            band |= pad[1 + dj:1 + dj + H, 1 + di:1 + di + W] != fate
+    if dirs is not None:
+        pd = np.pad(dirs, ((1, 1), (1, 1), (0, 0)), mode="edge")
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                nb = pd[1 + dj:1 + dj + H, 1 + di:1 + di + W]
+                ang = angle_between(dirs.reshape(-1, 3) + 1e-30, nb.reshape(-1, 3) + 1e-30).reshape(H, W)
+                band |= (fate == 0) & (ang > spread)
+    return band
+
+
+def compare(tag, gpu_rec, ora_rec, band):
+    """gpu_rec: (n, 12) records; ora_rec: (n, 16) oracle records."""
+    gf = gpu_rec[:, L.REC_FATE].astype(int)
+    of = ora_rec[:, 0].astype(int)
+    keep = ~band
+    fate_agree = np.mean(gf[keep] == of[keep])
+    esc = keep & (gf == 0) & (of == 0)
+    ang = angle_between(gpu_rec[esc, L.REC_DIR:L.REC_DIR + 3].astype(np.float64), ora_rec[esc, 2:5])
+    g_rgb = gpu_rec[:, L.REC_RGB:L.REC_RGB + 3].astype(np.float64)
+    o_rgb = ora_rec[:, 5:8]
+    both = keep & (gf == of)
+    rel = np.max(np.abs(g_rgb - o_rgb), axis=1) / np.maximum(np.max(np.abs(o_rgb), axis=1), 1e-6)
+    within = np.mean(rel[both] <= RAD_TOL)
+    ham = np.abs(gpu_rec[:, L.REC_HAM])
+    print(f"\n[{tag}] n={len(gf)} band={band.mean():.4f} fate_agree={fate_agree:.6f} "
+          f"dir_max={ang.max() if ang.size else 0:.3e} dir_p99={np.percentile(ang, 99) if ang.size else 0:.3e} "
+          f"rad_within_1e-3={within:.5f} rad_med={np.median(rel[both]):.2e} "
+          f"H_drift_med={np.median(ham[np.isfinite(ham)]):.2e} H_drift_p99={np.percentile(ham[np.isfinite(ham)], 99):.2e}",
+          flush=True)
+    return fate_agree, ang, within, rel[both]
+
+
+@pytest.fixture(scope="module")
+def c1_truth(oracle_lib):
+    sc = scenes.schwarzschild_320x180()
+    o = oracle_scene(sc)
+    ij = np.array([(i, j) for j in range(sc.height) for i in range(sc.width)], dtype=np.int32)
+    t0 = time.time()
+    rec = o.trace_list(ij, "tsit5", 1e-10)
+    print(f"\noracle tsit5 c1 full frame: {time.time() - t0:.1f}s", flush=True)
+    return sc, rec
+
+
+def test_c1_fp32_vs_tsit5_oracle(hip, c1_truth):
+    sc, orec = c1_truth
+    print("case: schwarzschild_320x180 fp32", flush=True)
+    img, rec, st = gpu_trace(sc)
+    g = rec.reshape(-1, L.REC_N)
+    ofate = orec[:, 0].reshape(sc.height, sc.width)
+    band = edge_band(ofate, orec[:, 2:5].reshape(sc.height, sc.width, 3)).ravel()
+    fa, ang, within, rel = compare("c1 fp32 vs tsit5", g, orec, band)
+    assert band.mean() < 0.03
+    assert fa == 1.0
+    assert ang.max() <= DIR_TOL
+    assert within >= 0.99
+    # the accumulated image is the sample radiance at weight 1
+    np.testing.assert_array_equal(img.reshape(-1, 3), g[:, L.REC_RGB:L.REC_RGB + 3])
+
+
+def test_c1_mixed_vs_tsit5_oracle(hip, c1_truth):
+    sc, orec = c1_truth
+    print("case: schwarzschild_320x180 mixed", flush=True)
+    img, rec, st = gpu_trace(sc, precision=L.ST_PREC_MIXED)
+    g = rec.reshape(-1, L.REC_N)
+    ofate = orec[:, 0].reshape(sc.height, sc.width)
+    band = edge_band(ofate, orec[:, 2:5].reshape(sc.height, sc.width, 3)).ravel()
+    fa, ang, within, rel = compare("c1 mixed vs tsit5", g, orec, band)
+    promoted = (g[:, L.REC_FLAGS].astype(int) & 1).astype(bool)
+    print(f"promoted fraction {promoted.mean():.4f} stats {st}")
+    assert st["promoted"] == promoted.sum() > 0
+    assert fa == 1.0 and ang.max() <= DIR_TOL and within >= 0.99
+
+
+def test_c1_fp64_matches_same_algorithm_oracle(hip, oracle_lib):
+    """All-FP64 GPU path vs the oracle's FP64 RK4 (identical algorithm):
+    agreement to FP64 round-off through the whole step sequence."""
+    sc = scenes.schwarzschild_320x180()
+    print("case: schwarzschild_320x180 fp64 vs oracle rk4", flush=True)
+    img, rec, st = gpu_trace(sc, precision=L.ST_PREC_FP64)
+    o = oracle_scene(sc)
+    ij = np.array([(i, j) for j in range(sc.height) for i in range(sc.width)], dtype=np.int32)
+    orec = o.trace_list(ij, "rk4")
+    g = rec.reshape(-1, L.REC_N).astype(np.float64)
+    same = g[:, 0] == orec[:, 0]
+    steps_same = g[:, 1] == orec[:, 1]
+    print(f"fate same {same.mean():.6f} steps same {steps_same.mean():.6f}")
+    assert same.mean() > 0.999 and steps_same.mean() > 0.999
+    ok = same & steps_same
+    np.testing.assert_allclose(g[ok, 5:8], orec[ok, 5:8], rtol=1e-5, atol=1e-7)
+
+
+@pytest.fixture(scope="module")
+def c2_truth(oracle_lib):
+    sc = scenes.kerr09_1080p()
+    o = oracle_scene(sc)
+    px = scenes.sample_pixels(sc.width, sc.height)
+    # oracle fate on each sample's 3x3 neighbourhood -> the edge band
+    offs = [(di, dj) for dj in (-1, 0, 1) for di in (-1, 0, 1)]
+    nb = np.array([(min(max(i + di, 0), sc.width - 1), min(max(j + dj, 0), sc.height - 1))
+                   for (i, j) in px for (di, dj) in offs], dtype=np.int32)
+    t0 = time.time()
+    rec9 = o.trace_list(nb, "tsit5", 1e-10).reshape(len(px), 9, 16)
+    print(f"\noracle tsit5 c2 4096x9 rays: {time.time() - t0:.1f}s", flush=True)
+    centre = rec9[:, 4, :]
+    fate9 = rec9[:, :, 0]
+    band = np.any(fate9 != fate9[:, 4:5], axis=1)
+    d9 = rec9[:, :, 2:5]
+    for k in range(9):
+        ang = angle_between(d9[:, k] + 1e-30, d9[:, 4] + 1e-30)
+        band |= (centre[:, 0] == 0) & (fate9[:, k] == 0) & (ang > 0.05)
+    return sc, px, centre, band
+
+
+@pytest.mark.parametrize("precision", [L.ST_PREC_FP32, L.ST_PREC_MIXED])
+def test_c2_kerr_1080p_sampled_vs_tsit5_oracle(hip, c2_truth, precision):
+    sc, px, orec, band = c2_truth
+    print(f"case: kerr09_1920x1080 precision={precision}", flush=True)
+    img, rec, st = gpu_trace(sc, precision=precision)
+    g = rec[px[:, 1], px[:, 0], :]
+    fa, ang, within, rel = compare(f"c2 prec{precision} vs tsit5 (4096 samples)", g, orec, band)
+    print(f"stats {st}")
+    assert band.mean() < 0.05
+    assert fa >= 0.999
+    assert np.percentile(ang, 99.9) <= DIR_TOL
+    assert within >= 0.985
+
+
+def test_shadow_radius_synge(hip):
+    """3 sqrt(3) M shadow seen by a static observer at r = 30M:
+    sin(alpha) = 3 sqrt(3) sqrt(1 - 2/r) / r; the GPU's captured disc edge
+    along the centre row and column lies within one pixel of it."""
+    r = 30.0
+    W = H = 1025
+    fov = 0.25
+    cam = api.Camera((0, -r, 0), (0, 0, 0), (0, 0, 1), fov)
+    disc = api.AccretionDisc(inner_radius=0.0, outer_radius=0.0)
+    sc = scenes.Scene("synge", api.Schwarzschild(1.0), cam, disc, api.StepRule.fixed_fraction(0.05, 200.0, 1.01),
+                      W, H)
+    for prec in (L.ST_PREC_FP32, L.ST_PREC_MIXED):
+        print(f"case: synge shadow precision={prec}", flush=True)
+        img, rec, st = gpu_trace(sc, precision=prec)
+        fate = rec[:, :, L.REC_FATE]
+        want = math.asin(3 * math.sqrt(3) * math.sqrt(1 - 2 / r) / r)
+        c = W // 2
+        radii = []
+        for line in (fate[c, :], fate[:, c]):
+            cap = np.nonzero(line == 1)[0]
+            lo, hi = cap.min(), cap.max()
+            for edge_px, outward in ((hi, +1), (lo, -1)):
+                # boundary between the last captured pixel centre and the next
+                v_in = (edge_px + 0.5 - W / 2) / (H / 2)
+                v_out = (edge_px + outward + 0.5 - W / 2) / (H / 2)
+                a_in, a_out = math.atan(abs(v_in) * fov), math.atan(abs(v_out) * fov)
+                assert a_in <= want + 1e-12 <= a_out + 1e-12 or abs(a_in - want) < 1e-4, (a_in, want, a_out)
+                radii.append(0.5 * (a_in + a_out))
+        print(f"synge want {want:.6f} got {np.mean(radii):.6f} (pixel {math.atan(fov / (H / 2)):.2e} rad)")
+
+
+def test_reference_rule_fp32_vs_fp64_oracle(hip, oracle_lib):
+    """The reference's own GPU rule (dt = 0.02 draft step, kill radii,
+    Kerr step shrink, Bardeen verdict, direction at the end state): FP32 GPU
+    against the FP64 oracle running the identical algorithm."""
+    sc = scenes.kerr09_1080p(480, 270)
+    sc.rule = api.StepRule.reference(0.02)
+    sc.disc = api.AccretionDisc()  # reference disc: 22.8 kK (r/2M)^-3/4, ISCO hold
+    print("case: kerr09 480x270 reference rule fp32", flush=True)
+    img, rec, st = gpu_trace(sc)
+    o = oracle_scene(sc)
+    ij = np.array([(i, j) for j in range(sc.height) for i in range(sc.width)], dtype=np.int32)
+    orec = o.trace_list(ij, "rk4")
+    band = edge_band(orec[:, 0].reshape(sc.height, sc.width), orec[:, 2:5].reshape(sc.height, sc.width, 3)).ravel()
+    fa, ang, within, rel = compare("reference rule fp32 vs fp64 same-algorithm", rec.reshape(-1, L.REC_N), orec,
+                                   band)
+    assert fa >= 0.999 and np.percentile(ang, 99.9) <= DIR_TOL and within >= 0.98
+
+
+RAYGEN_CASES = {
+    "fisheye": dict(fisheye_deg=80.0),
+    "boosted": dict(velocity=(0.0, 0.4, -0.1)),
+    "thin_lens": dict(lens=True),
+    "jitter": dict(jw=0.5, jseed=77),
+    "shutter": dict(shutter=True),
+    "inside_horizon": dict(pos=(0.0, -1.5, 0.2)),
+}
+
+
+@pytest.mark.parametrize("case", list(RAYGEN_CASES))
+def test_raygen_variants_fp64_vs_oracle(hip, oracle_lib, case):
+    """Ray generation variants (src/metal.jl:833-995) in the FP64 kernel vs
+    the oracle's FP64 RK4: same pixels, same hashes, same algorithm."""
+    kw = RAYGEN_CASES[case]
+    pos = kw.get("pos", (0.0, -30.0, 3.0))
+    a = 0.0 if case == "inside_horizon" else 0.9
+    st = api.Kerr(1.0, a) if a else api.Schwarzschild(1.0)
+    cam = api.Camera(pos, (0, 0, 0), (0, 0, 1), 0.75, velocity=kw.get("velocity", (0, 0, 0)))
+    W, H = 96, 64
+    sc = scenes.Scene(case, st, cam, scenes.config_disc(), api.StepRule.fixed_fraction(0.05, 100.0, 1.01), W, H)
+    c32, c64 = api.ks_cam_params(cam, 1.0, a, fisheye_deg=kw.get("fisheye_deg", 0.0), focus_dist=12.0)
+    if kw.get("lens"):
+        for arr in (c32, c64):
+            arr[22], arr[23], arr[25], arr[26], arr[27] = 0.25, 0.5, 0.5, 0.4, 3.0
+    if kw.get("shutter"):
+        cam2 = api.Camera((0.5, -29.0, 3.3), (0, 0, 0), (0, 0, 1), 0.75)
+        e32, e64 = api.ks_cam_params(cam2, 1.0, a)
+        c32[28] = c64[28] = 104731.0
+        c32[29:49] = e32[0:20]
+        c64[29:49] = e64[0:20]
+    print(f"case: raygen {case}", flush=True)
+    ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=sc.disc)
+    ctx.enable_records(True)
+    sp = api.spacetime_params(st, cam, sc.rule)
+    args = api.trace_args(W, H, sc.rule, precision=L.ST_PREC_FP64, jw=kw.get("jw", 0.0), jseed=kw.get("jseed", 0),
+                          ju=0.25 if "jw" in kw else 0.5)
+    ctx.clear()
+    ctx.trace(c32, c64, sp, args)
+    rec = ctx.records().reshape(-1, L.REC_N).astype(np.float64)
+    ctx.close()
+    o = oracle_scene(sc, cam=c64, jw=kw.get("jw", 0.0), jseed=kw.get("jseed", 0), ju=0.25 if "jw" in kw else 0.5)
+    ij = np.array([(i, j) for j in range(H) for i in range(W)], dtype=np.int32)
+    orec = o.trace_list(ij, "rk4")
+    same = (rec[:, 0] == orec[:, 0]) & (rec[:, 1] == orec[:, 1])
+    print(f"[{case}] identical fate+steps {same.mean():.5f}")
+    assert same.mean() >= 0.995
+    np.testing.assert_allclose(rec[same, 5:8], orec[same, 5:8], rtol=1e-5, atol=1e-6)
+    esc = same & (rec[:, 0] == 0)
+    assert angle_between(rec[esc, 2:5], orec[esc, 2:5]).max() < 1e-6
+
+
+def test_nonfinite_camera_paints_black(hip):
+    """NaN in the camera block: every ray is non-finite, painted black, fate 2
+    — a pixel value, never an error (src/metal.jl:1414-1421)."""
+    sc = scenes.schwarzschild_320x180()
+    W, H = 64, 32
+    ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=sc.disc)
+    ctx.enable_records(True)
+    c32, c64 = sc.cam_params()
+    c32[5] = np.nan
+    c64 = None
+    sp = sc.spacetime_params()
+    ctx.clear()
+    ctx.trace(c32, c64, sp, api.trace_args(W, H, sc.rule))
+    img = ctx.download()
+    rec = ctx.records()
+    ctx.close()
+    assert np.all(img == 0.0)
+    assert np.all(rec[:, :, L.REC_FATE] == 2)
+
+
+def test_bad_arguments_are_error_codes(hip):
+    sc = scenes.schwarzschild_320x180()
+    ctx = api.HipPreviewContext(scenes.sky(), 32, 16, disc=sc.disc)
+    c32, c64 = sc.cam_params()
+    sp = sc.spacetime_params()
+    bad = sp.copy(); bad[6] = 1.5
+    with pytest.raises(L.StError) as e:
+        ctx.trace(c32, c64, bad, api.trace_args(32, 16, sc.rule))
+    assert e.value.code == L.ST_ERR_INVALID_ARG
+    with pytest.raises(L.StError):
+        ctx.trace(c32, c64, sp, api.trace_args(64, 16, sc.rule))   # size mismatch
+    args = api.trace_args(32, 16, sc.rule)
+    args.order = 45
+    with pytest.raises(L.StError) as e:
+        ctx.trace(c32, c64, sp, args)
+    assert e.value.code == L.ST_ERR_UNSUPPORTED
+    ctx.close()
