@@ -180,6 +180,63 @@ int st_trace_batch(st_ctx* ctx, const st_trace_args* args, int tile, int waves_p
 /* Same, captured once into a hipGraph and replayed on later calls. */
 int st_trace_batch_graph(st_ctx* ctx, const st_trace_args* args, int tile, int waves_per_cu);
 int st_download_batch(st_ctx* ctx, float* host_out, size_t nfloats);
+/* Autotuner for the batched launch: times every (wave tile width in
+ * {8, 16, 32, 64}) x (waves per CU in {2, 4, 8, 12, 16}) shape on the
+ * current batch, keeps the fastest on the context (used when
+ * st_trace_batch gets tile = 0) and, if cache_path is not NULL, records it
+ * in that text file under a key of device name, frame count, pixel count,
+ * spin and precision.  A later context finds the entry there without
+ * re-timing.  No counterpart in the reference. */
+int st_autotune_batch(st_ctx* ctx, const st_trace_args* args, const char* cache_path, int* best_tile,
+                      int* best_waves, float* best_ms);
+
+/* ---- learned surrogate of the lensing map ----
+ * No counterpart in the reference.  An MLP 5 -> 256 -> 256 -> 256 -> 256 -> 5
+ * (SiLU) maps (r_cam, theta_cam, a, alpha, beta) — a static/ZAMO camera at
+ * (sqrt(r^2+a^2) sin theta, 0, r cos theta) looking at the hole with up +z,
+ * and a ray with tan(alpha), tan(beta) its right/up over forward components —
+ * to the escape direction (as a correction to the undeflected direction, in
+ * camera axes) and two fate logits (escaped, captured).  BF16 weights and
+ * activations, FP32 accumulation on MFMA; trained on rays the RK4 kernel
+ * traces.  The parameters are one flat FP32 array of st_mlp_param_count()
+ * floats: W1[256][5], b1[256], W2..W4[256][256] + b, W5[64][256] (rows >= 5
+ * zero), b5[64]. */
+typedef struct st_mlp st_mlp;
+
+typedef struct {
+    int n_cameras;         /* cameras drawn from the domain */
+    int side;              /* each camera traces side x side jittered rays */
+    float r_min, r_max;    /* camera KS radius (M units) */
+    float a_min, a_max;    /* spin */
+    float fov;             /* tan of the largest ray angle alpha, beta */
+    uint64_t seed;
+} st_surrogate_domain;
+
+typedef struct {
+    int steps, batch, warmup;
+    float lr, lr_final;    /* cosine schedule after linear warm-up */
+    float w_dir;           /* weight of the direction loss against the fate loss */
+    float clip;            /* global gradient-norm clip (<= 0: off) */
+    uint64_t seed;
+} st_train_opts;
+
+size_t st_mlp_param_count(void);
+int st_mlp_create(st_mlp** out, uint64_t seed);
+int st_mlp_destroy(st_mlp* mlp);
+int st_mlp_get_params(st_mlp* mlp, float* host, size_t n);
+int st_mlp_set_params(st_mlp* mlp, const float* host, size_t n);
+/* One forward + backward pass (no update) on n samples (n % 64 == 0):
+ * x[n][5] features, t[n][8] = (d_cam xyz, fate, valid, n_cam xyz).  Writes the
+ * outputs y[n][5], the gradient of the mean loss (st_mlp_param_count floats)
+ * and the loss sums (direction, fate, count). */
+int st_mlp_grad(st_mlp* mlp, const float* x, const float* t, int n, float* y_out, float* grad_out, float w_dir,
+                float* loss_out);
+/* Trace n_cameras * side^2 rays with the RK4 kernel (disc off) over the
+ * domain and keep them on the device as the training set. */
+int st_mlp_generate(st_mlp* mlp, st_ctx* ctx, const st_surrogate_domain* dom, const float spacetime_params[8],
+                    const st_trace_args* args);
+/* Adam steps on random mini-batches; loss_out[steps][3] (may be NULL). */
+int st_mlp_train(st_mlp* mlp, const st_train_opts* opts, float* loss_out);
 
 /* ---- host helpers (FP64 host math, no device work) ---- */
 /* _ks_cam_params (src/metal.jl:1760-1774) for Camera(pos, target, up, fov;

@@ -1,0 +1,112 @@
+"""FP64 numpy restatement of the surrogate MLP's forward pass, loss and
+backward pass — TEST INFRASTRUCTURE ONLY (the checker for the GPU's BF16
+MFMA training kernels).
+
+Network: 5 -> 256 -> 256 -> 256 -> 256 -> 5, SiLU on the hidden layers.
+Parameters: one flat array in the library's order
+    W1[256][5], b1[256], W2[256][256], b2, W3, b3, W4, b4, W5[64][256], b5[64]
+(output rows >= 5 are padding and stay zero).
+Loss per batch of B samples, targets t = (d xyz, fate, valid, n xyz):
+    L = (1/B) sum_valid [ w_dir * [fate == 0] * |y[0:3] + n - d|^2 / 3
+                          + CE(softmax(y[3:5]), fate) ]
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NIN, HID, NOUT, NOUTP = 5, 256, 5, 64
+
+
+def offsets():
+    o = {}
+    p = 0
+    for name, size in (("W1", HID * NIN), ("b1", HID), ("W2", HID * HID), ("b2", HID), ("W3", HID * HID),
+                       ("b3", HID), ("W4", HID * HID), ("b4", HID), ("W5", NOUTP * HID), ("b5", NOUTP)):
+        o[name] = (p, size)
+        p += size
+    o["N"] = p
+    return o
+
+
+OFF = offsets()
+
+
+def unpack(P):
+    P = np.asarray(P, dtype=np.float64)
+    g = lambda k: P[OFF[k][0]:OFF[k][0] + OFF[k][1]]
+    return dict(W1=g("W1").reshape(HID, NIN), b1=g("b1"), W2=g("W2").reshape(HID, HID), b2=g("b2"),
+                W3=g("W3").reshape(HID, HID), b3=g("b3"), W4=g("W4").reshape(HID, HID), b4=g("b4"),
+                W5=g("W5").reshape(NOUTP, HID)[:NOUT], b5=g("b5")[:NOUT])
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def silu_grad(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(P, X):
+    w = unpack(P)
+    Z1 = X @ w["W1"].T + w["b1"]; H1 = silu(Z1)
+    Z2 = H1 @ w["W2"].T + w["b2"]; H2 = silu(Z2)
+    Z3 = H2 @ w["W3"].T + w["b3"]; H3 = silu(Z3)
+    Z4 = H3 @ w["W4"].T + w["b4"]; H4 = silu(Z4)
+    Y = H4 @ w["W5"].T + w["b5"]
+    return Y, (X, Z1, H1, Z2, H2, Z3, H3, Z4, H4)
+
+
+def loss_and_dY(Y, T, w_dir):
+    B = Y.shape[0]
+    d, fate, valid, n = T[:, 0:3], T[:, 3].astype(int), T[:, 4] > 0, T[:, 5:8]
+    e = Y[:, 0:3] + n - d
+    esc = valid & (fate == 0)
+    ldir = w_dir * np.sum(e[esc] ** 2) / 3.0
+    m = np.max(Y[:, 3:5], axis=1, keepdims=True)
+    p = np.exp(Y[:, 3:5] - m)
+    p /= p.sum(axis=1, keepdims=True)
+    onehot = np.stack([fate == 0, fate == 1], axis=1).astype(np.float64)
+    lfate = -np.sum(np.log(np.maximum(np.sum(p * onehot, axis=1), 1e-300))[valid])
+    dY = np.zeros_like(Y)
+    dY[esc, 0:3] = w_dir * 2.0 * e[esc] / 3.0 / B
+    dY[valid, 3:5] = (p - onehot)[valid] / B
+    return (ldir + lfate) / B, ldir, lfate, dY
+
+
+def backward(P, cache, dY):
+    w = unpack(P)
+    X, Z1, H1, Z2, H2, Z3, H3, Z4, H4 = cache
+    g = np.zeros(OFF["N"])
+
+    def put(k, v):
+        a, n = OFF[k]
+        flat = v.ravel()
+        g[a:a + flat.size] = flat
+
+    gW5 = np.zeros((NOUTP, HID)); gW5[:NOUT] = dY.T @ H4
+    put("W5", gW5)
+    gb5 = np.zeros(NOUTP); gb5[:NOUT] = dY.sum(0)
+    put("b5", gb5)
+    dH = dY @ w["W5"]
+    for l, (Z, Hprev) in ((4, (Z4, H3)), (3, (Z3, H2)), (2, (Z2, H1))):
+        dZ = dH * silu_grad(Z)
+        put(f"W{l}", dZ.T @ Hprev)
+        put(f"b{l}", dZ.sum(0))
+        dH = dZ @ w[f"W{l}"]
+    dZ1 = dH * silu_grad(Z1)
+    put("W1", dZ1.T @ X)
+    put("b1", dZ1.sum(0))
+    return g
+
+
+def loss_grad(P, X, T, w_dir):
+    Y, cache = forward(P, X)
+    L, ldir, lfate, dY = loss_and_dY(Y, T, w_dir)
+    return L, backward(P, cache, dY), Y
+
+
+def loss_only(P, X, T, w_dir):
+    Y, _ = forward(P, X)
+    return loss_and_dY(Y, T, w_dir)[0]

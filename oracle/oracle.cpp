@@ -761,10 +761,18 @@ void finish(const ora_scene* sc, const double s[6], double p_t, double scam, boo
     for (int c = 0; c < 3; ++c) o->rgb[c] = sh.acc[c] + sh.alpha * col[c];
 }
 
+inline double ks_r2(const ora_scene* sc, const double s[6]) {
+    if (!sc->kerr) return s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double a2 = sc->a * sc->a;
+    const double aw = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] - a2;
+    return 0.5 * (aw + std::sqrt(aw * aw + 4.0 * a2 * s[2] * s[2]));
+}
+
 struct RayStart {
     double s[6];
     double p_t, scam, isco[3], rplus, rcap, r_floor;
-    bool captured0;
+    bool captured0, no_sky;
+    double rk2_prev0;
 };
 
 void ray_start(const ora_scene* sc, int i, int j, ora_ray* o, RayStart& rs) {
@@ -777,16 +785,15 @@ void ray_start(const ora_scene* sc, int i, int j, ora_ray* o, RayStart& rs) {
     rs.captured0 = sc->kerr && sc->bardeen && sc->a != 0.0 && o->captured0;
     rs.rplus = ora_horizon_radius(sc->M, sc->kerr ? sc->a : 0.0);
     rs.rcap = sc->capture_factor * rs.rplus;
+    /* capture mode 1: below the capture radius and moving inward; a camera
+     * inside it starts with rays moving outward */
+    rs.rk2_prev0 = ks_r2(sc, rs.s) > rs.rcap * rs.rcap ? 1e300 : 0.0;
+    /* E = p_t <= 0: the ray cannot have come from infinity (capture mode 1) */
+    rs.no_sky = sc->capture_mode == 1 && !(rs.p_t > 0.0);
     const double r0 = std::sqrt(rs.s[0] * rs.s[0] + rs.s[1] * rs.s[1] + rs.s[2] * rs.s[2]);
     rs.r_floor = r0 > 2.05 * sc->M ? 2.0 * sc->M : 0.3 * sc->M;
 }
 
-inline double ks_r2(const ora_scene* sc, const double s[6]) {
-    if (!sc->kerr) return s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
-    const double a2 = sc->a * sc->a;
-    const double aw = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] - a2;
-    return 0.5 * (aw + std::sqrt(aw * aw + 4.0 * a2 * s[2] * s[2]));
-}
 
 }  // namespace
 
@@ -802,8 +809,9 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
     double s[6];
     std::memcpy(s, rs.s, sizeof(s));
     Shade sh;
-    bool hit = false, nonfinite = false, escaped_loop = false;
+    bool hit = rs.no_sky, nonfinite = false, escaped_loop = false;
     double r_prev = -1.0;
+    double rk2_prev = rs.rk2_prev0;
     double xstart[3], vstart[3];
     bool have_prev = false;
     o->rk_min = 1e30;
@@ -815,13 +823,14 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
     const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
     const double ds_gas = mstep * sc->dt;
     int steps = 0;
-    for (int it = 0; it < sc->nmax; ++it) {
+    for (int it = 0; it < (rs.no_sky ? 0 : sc->nmax); ++it) {
         const double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
         const double r = std::sqrt(r2);
         const double rk2 = ks_r2(sc, s);
         o->rk_min = std::min(o->rk_min, std::sqrt(std::max(rk2, 0.0)));
         if (sc->capture_mode == 1) {
-            if (rk2 < rs.rcap * rs.rcap) { hit = true; break; }
+            if ((rk2 < rs.rcap * rs.rcap && rk2 < rk2_prev) || rk2 < 0.09 * M * M) { hit = true; break; }
+            rk2_prev = rk2;
         } else if (sc->kerr) {
             if (rk2 < sc->rkill2) { hit = true; break; }
         } else if (r < 3.2 * M) {
@@ -855,7 +864,7 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
         stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
         ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k4);
         o->rhs_evals += 4;
-        if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap) { hit = true; break; }
+        if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap && stage_rk2 < rk2) { hit = true; break; }
         for (int c = 0; c < 6; ++c) s[c] += (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
         ++steps;
         for (int c = 0; c < 3; ++c) { xstart[c] = sp[c]; vstart[c] = k1[c]; }
@@ -972,8 +981,13 @@ int ora_trace_tsit5(const ora_scene* sc, int i, int j, double rtol, double atol,
     const double vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
     const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
     const double ds_gas = mstep * sc->dt;
-    bool hit = false, nonfinite = false;
+    bool hit = rs.no_sky, nonfinite = false;
     o->rk_min = std::sqrt(std::max(ks_r2(sc, s), 0.0));
+    if (hit) {
+        o->steps = 0;
+        finish(sc, s, rs.p_t, rs.scam, true, false, rs.captured0, nullptr, nullptr, sh, o);
+        return o->fate;
+    }
     const double r0 = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     if (r0 > sc->r_escape) {  /* camera outside the escape sphere */
         finish(sc, s, rs.p_t, rs.scam, false, false, rs.captured0, nullptr, nullptr, sh, o);
@@ -1075,9 +1089,14 @@ int ora_trace_tsit5(const ora_scene* sc, int i, int j, double rtol, double atol,
             march_gas(sc, s, snew, hused * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
             if (sh.alpha < 0.003) { std::memcpy(s, snew, sizeof(s)); break; }
         }
+        const double rk2_old = ks_r2(sc, s);
         std::memcpy(s, snew, sizeof(s));
         o->rk_min = std::min(o->rk_min, std::sqrt(std::max(ks_r2(sc, s), 0.0)));
         if (which == 0) { hit = true; break; }
+        if (sc->capture_mode == 1) {
+            const double rk2_new = ks_r2(sc, s);
+            if (rk2_new < rs.rcap * rs.rcap && rk2_new < rk2_old) { hit = true; break; }
+        }
         if (which == 1) { done = true; break; }
         if (which == 2) {
             s[2] = 0.0;  /* exactly on the plane */

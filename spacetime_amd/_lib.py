@@ -55,6 +55,16 @@ class FrameDesc(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("out_offset", C.c_int)]
 
 
+class SurrogateDomain(C.Structure):
+    _fields_ = [("n_cameras", C.c_int), ("side", C.c_int), ("r_min", C.c_float), ("r_max", C.c_float),
+                ("a_min", C.c_float), ("a_max", C.c_float), ("fov", C.c_float), ("seed", C.c_uint64)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("steps", C.c_int), ("batch", C.c_int), ("warmup", C.c_int), ("lr", C.c_float),
+                ("lr_final", C.c_float), ("w_dir", C.c_float), ("clip", C.c_float), ("seed", C.c_uint64)]
+
+
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
 
@@ -80,6 +90,16 @@ SIGNATURES = {
     "st_trace_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),
     "st_trace_batch_graph": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),
     "st_download_batch": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_char_p, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "st_mlp_param_count": (C.c_size_t, []),
+    "st_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
+    "st_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "st_mlp_get_params": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_mlp_set_params": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_mlp_grad": (C.c_int, [C.c_void_p, _F, _F, C.c_int, _F, _F, C.c_float, _F]),
+    "st_mlp_generate": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateDomain), _F, C.POINTER(TraceArgs)]),
+    "st_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), _F]),
     "st_camera_params": (C.c_int, [_D, _D, _D, C.c_double, _D, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, _D]),
     "st_camera_params_basis": (C.c_int, [_D, _D, _D, _D, C.c_double, _D, C.c_double, C.c_double, _D]),

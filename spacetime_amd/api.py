@@ -481,3 +481,37 @@ def render(cam: Camera, spacetime, background: np.ndarray, *, disc: Optional[Acc
                                 pixel_jitter=samples > 1)
     finally:
         ctx.close()
+
+
+class BatchTrace:
+    """Many frames of uneven size traced by one persistent launch (optionally
+    replayed from a captured hipGraph), with an autotuned launch shape."""
+
+    def __init__(self, ctx: HipPreviewContext, cams32: np.ndarray, cams64: Optional[np.ndarray], sizes, sp):
+        descs, off = [], 0
+        for (w, h) in sizes:
+            descs.append((int(w), int(h), off))
+            off += 3 * int(w) * int(h)
+        self.ctx, self.sizes, self.total = ctx, [(int(w), int(h)) for w, h in sizes], off
+        self.offsets = [d[2] for d in descs]
+        arr = (L.FrameDesc * len(descs))(*[L.FrameDesc(*d) for d in descs])
+        c32 = np.ascontiguousarray(cams32, dtype=np.float32)
+        c64 = None if cams64 is None else np.ascontiguousarray(cams64, dtype=np.float64)
+        L.check(L.lib().st_batch_create(ctx.handle, len(descs), arr, L.fptr(c32), L.dptr(c64),
+                                        L.fptr(np.ascontiguousarray(sp, dtype=np.float32))))
+
+    def trace(self, args: L.TraceArgs, tile: int = 0, waves_per_cu: int = 0, graph: bool = False):
+        fn = L.lib().st_trace_batch_graph if graph else L.lib().st_trace_batch
+        L.check(fn(self.ctx.handle, C.byref(args), int(tile), int(waves_per_cu)))
+
+    def autotune(self, args: L.TraceArgs, cache_path: Optional[str] = None):
+        t, w, ms = C.c_int(), C.c_int(), C.c_float()
+        L.check(L.lib().st_autotune_batch(self.ctx.handle, C.byref(args),
+                                          cache_path.encode() if cache_path else None, C.byref(t), C.byref(w),
+                                          C.byref(ms)))
+        return t.value, w.value, ms.value
+
+    def download(self) -> list:
+        flat = np.empty(self.total, dtype=np.float32)
+        L.check(L.lib().st_download_batch(self.ctx.handle, L.fptr(flat), self.total))
+        return [flat[o:o + 3 * w * h].reshape(h, w, 3) for o, (w, h) in zip(self.offsets, self.sizes)]

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,77 +18,17 @@
 using st::DeviceScene;
 using st::FrameRef;
 using st::TraceParams;
+using st::fail;
+using st::make_scene;
+using st::fill_params;
 
-namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
+int st::fail(int code, const std::string& msg) {
+    st::last_error() = msg;
     return code;
 }
 
-#define ST_HIP(call)                                                                                        \
-    do {                                                                                                    \
-        hipError_t e_ = (call);                                                                             \
-        if (e_ != hipSuccess)                                                                               \
-            return fail(e_ == hipErrorOutOfMemory ? ST_ERR_OOM : ST_ERR_DEVICE,                             \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                                 \
-    } while (0)
 
-struct GraphCache {
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    int tw = 0, nblocks = 0;
-    st_trace_args args{};
-    bool valid = false;
-};
-
-}  // namespace
-
-struct st_ctx {
-    int device = 0;
-    int num_cu = 256;
-    hipStream_t stream = nullptr;
-    int width = 0, height = 0;
-    float4* sky = nullptr;
-    int sky_w = 0, sky_h = 0;
-    float4* lut = nullptr;
-    int lut_n = 1;
-    float disc_params[6] = {0, 0, 0, 500.f, 30000.f, 1.f};
-    float T0 = 22800.0f, Tr0 = 2.0f;
-    int T_hold = 1;
-    bool T_from_M = true;   // reference law: r0 = 2M unless set explicitly
-    float* vol = nullptr;
-    int nr = 2, nphi = 1, nz = 2;
-    float vol_params[9] = {0, 0, 0, 0, 0, 0, 0, 0, 2.0f};
-    bool has_volume = false, vol_on = false;
-    float4* star_lut = nullptr;
-    int star_lut_n = 1;
-    float star_params[16] = {0};
-    float* out = nullptr;
-    float* rec = nullptr;
-    bool rec_on = false;
-    uint2* queue = nullptr;
-    size_t queue_cap = 0;
-    unsigned* qcount = nullptr;
-    unsigned long long* counters = nullptr;
-    unsigned* work_next = nullptr;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    bool last_mixed = false;
-    bool have_last = false;
-    // batch
-    int nframes = 0;
-    std::vector<FrameRef> frames_host;
-    FrameRef* frames_dev = nullptr;
-    float* batch_out = nullptr;
-    size_t batch_out_n = 0;
-    int* tile_base_dev = nullptr;
-    int tile_base_tw = 0;
-    int total_items = 0;
-    float batch_st[8] = {0};
-    GraphCache graph;
-};
 
 namespace {
 
@@ -107,7 +48,9 @@ int upload_float4(float4** dst, const float* src3, int n) {
     return ST_OK;
 }
 
-DeviceScene make_scene(const st_ctx* c) {
+}  // namespace
+
+DeviceScene st::make_scene(const st_ctx* c) {
     DeviceScene D;
     D.sky = c->sky; D.sky_w = c->sky_w; D.sky_h = c->sky_h;
     D.lut = c->lut; D.lut_n = std::max(c->lut_n, 1);
@@ -123,12 +66,16 @@ DeviceScene make_scene(const st_ctx* c) {
     return D;
 }
 
+namespace {
+
 int ref_nmax(float r_escape, float M, float dt) {
     const double n = std::ceil((75.0 + 6.5 * std::log(std::max((double)r_escape / M, 1.0))) * M / dt);
     return (int)std::min(std::max(1000.0, n), 40000.0);
 }
 
-int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P) {
+}  // namespace
+
+int st::fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P) {
     if (!(sp[0] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "spacetime_params[0] (M) must be > 0");
     if (std::fabs(sp[6]) > sp[0] * (1.0f + 1e-6f)) return fail(ST_ERR_INVALID_ARG, "|a| must be <= M");
     if (!(a->dt > 0.0f)) return fail(ST_ERR_INVALID_ARG, "dt must be > 0");
@@ -150,6 +97,8 @@ int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParam
     return ST_OK;
 }
 
+namespace {
+
 int ensure_queue(st_ctx* c, size_t n) {
     if (c->queue_cap >= n) return ST_OK;
     free_dev(c->queue);
@@ -167,7 +116,7 @@ void sync_temperature_scale(st_ctx* c, float M) {
 
 extern "C" {
 
-const char* st_last_error(void) { return g_err.c_str(); }
+const char* st_last_error(void) { return st::last_error().c_str(); }
 const char* st_version(void) { return "libspacetime_hip 0.1 (gfx950)"; }
 
 void st_trace_args_default(st_trace_args* a, int width, int height) {
@@ -454,12 +403,16 @@ int st_batch_create(st_ctx* c, int nframes, const st_frame_desc* fr, const float
     ST_HIP(hipMalloc(&c->frames_dev, sizeof(FrameRef) * nframes));
     ST_HIP(hipMemcpy(c->frames_dev, c->frames_host.data(), sizeof(FrameRef) * nframes, hipMemcpyHostToDevice));
     c->nframes = nframes;
+    c->batch_pixels = npix;
     std::memcpy(c->batch_st, sp, sizeof(c->batch_st));
     c->tile_base_tw = 0;
+    c->tuned_tile = c->tuned_waves = 0;
     const int rc = ensure_queue(c, npix);
     if (rc != ST_OK) return rc;
     return ST_OK;
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -504,19 +457,56 @@ int enqueue_batch(st_ctx* c, const st_trace_args* a, int tw, int nblocks) {
     return ST_OK;
 }
 
-void resolve_shape(st_ctx* c, int& tw, int& waves) {
+std::string tune_key(st_ctx* c, const st_trace_args* a) {
+    hipDeviceProp_t prop;
+    std::string dev = "unknown";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) dev = prop.gcnArchName;
+    for (char& ch : dev)
+        if (ch == ' ' || ch == ':') ch = '_';
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s/cu%d/f%d/p%zu/a%.4f/prec%d/ord%d", dev.c_str(), c->num_cu, c->nframes,
+                  c->batch_pixels, (double)c->batch_st[6], a->precision, a->order);
+    return buf;
+}
+
+bool tune_lookup(const char* path, const std::string& key, int& tw, int& waves) {
+    if (!path) return false;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    char k[512];
+    int t = 0, w = 0;
+    float ms = 0;
+    bool found = false;
+    while (std::fscanf(f, "%511s %d %d %f", k, &t, &w, &ms) == 4) {
+        if (key == k) { tw = t; waves = w; found = true; }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void resolve_shape(st_ctx* c, const st_trace_args* a, int& tw, int& waves) {
+    if (tw == 0 || waves <= 0) {
+        int t = c->tuned_tile, w = c->tuned_waves;
+        if (!(t && w)) {
+            const char* path = std::getenv("ST_TUNE_CACHE");
+            if (!tune_lookup(path, tune_key(c, a), t, w)) { t = 8; w = 8; }
+        }
+        if (tw == 0) tw = t;
+        if (waves <= 0) waves = w;
+    }
     if (tw != 8 && tw != 16 && tw != 32 && tw != 64) tw = 8;
-    if (waves <= 0) waves = 8;
-    (void)c;
+    if (waves > 32) waves = 32;
 }
 
 }  // namespace
+
+extern "C" {
 
 int st_trace_batch(st_ctx* c, const st_trace_args* a, int tile, int waves_per_cu) {
     if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
     int tw = tile, waves = waves_per_cu;
-    resolve_shape(c, tw, waves);
+    resolve_shape(c, a, tw, waves);
     int rc = prepare_tiles(c, tw);
     if (rc != ST_OK) return rc;
     const int nblocks = std::max(1, c->num_cu * waves / 4);
@@ -534,11 +524,11 @@ int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_
     if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
     int tw = tile, waves = waves_per_cu;
-    resolve_shape(c, tw, waves);
+    resolve_shape(c, a, tw, waves);
     int rc = prepare_tiles(c, tw);
     if (rc != ST_OK) return rc;
     const int nblocks = std::max(1, c->num_cu * waves / 4);
-    GraphCache& g = c->graph;
+    st::GraphCache& g = c->graph;
     const bool same = g.valid && g.tw == tw && g.nblocks == nblocks && std::memcmp(&g.args, a, sizeof(*a)) == 0;
     if (!same) {
         if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
@@ -563,6 +553,45 @@ int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_
     return ST_OK;
 }
 
+int st_autotune_batch(st_ctx* c, const st_trace_args* a, const char* cache_path, int* best_tile, int* best_waves,
+                      float* best_ms) {
+    if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
+    static const int tiles[] = {8, 16, 32, 64};
+    static const int waves[] = {2, 4, 8, 12, 16};
+    float best = 1e30f;
+    int bt = 8, bw = 8;
+    for (int tw : tiles) {
+        for (int w : waves) {
+            int rc = st_trace_batch(c, a, tw, w);   // warm-up (and tile table)
+            if (rc != ST_OK) return rc;
+            float tot = 0.0f;
+            for (int rep = 0; rep < 2; ++rep) {
+                rc = st_trace_batch(c, a, tw, w);
+                if (rc != ST_OK) return rc;
+                ST_HIP(hipEventSynchronize(c->ev[2]));
+                float ms = 0.0f;
+                ST_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[2]));
+                tot += ms;
+            }
+            const float ms = 0.5f * tot;
+            if (ms < best) { best = ms; bt = tw; bw = w; }
+        }
+    }
+    c->tuned_tile = bt;
+    c->tuned_waves = bw;
+    if (best_tile) *best_tile = bt;
+    if (best_waves) *best_waves = bw;
+    if (best_ms) *best_ms = best;
+    if (cache_path) {
+        FILE* f = std::fopen(cache_path, "a");
+        if (!f) return fail(ST_ERR_INVALID_ARG, std::string("cannot write tune cache ") + cache_path);
+        std::fprintf(f, "%s %d %d %.4f\n", tune_key(c, a).c_str(), bt, bw, best);
+        std::fclose(f);
+    }
+    return ST_OK;
+}
+
 int st_download_batch(st_ctx* c, float* host, size_t n) {
     if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (n > c->batch_out_n) return fail(ST_ERR_INVALID_ARG, "more floats requested than the batch holds");
@@ -572,3 +601,8 @@ int st_download_batch(st_ctx* c, float* host, size_t n) {
 }
 
 }  // extern "C"
+
+std::string& st::last_error() {
+    static thread_local std::string e;
+    return e;
+}

@@ -2,16 +2,93 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+#include <vector>
+
+#include "spacetime_hip.h"
+#include "st_trace.h"
+
 namespace st {
 
-struct DeviceScene;
-struct TraceParams;
-struct FrameRef;
+std::string& last_error();
+int fail(int code, const std::string& msg);
+
+#define ST_HIP(call)                                                                                        \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return st::fail(e_ == hipErrorOutOfMemory ? ST_ERR_OOM : ST_ERR_DEVICE,                         \
+                            std::string(#call) + ": " + hipGetErrorString(e_));                             \
+    } while (0)
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int tw = 0, nblocks = 0;
+    st_trace_args args{};
+    bool valid = false;
+};
+
+}  // namespace st
+
+// The opaque context of the C ABI.
+struct st_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    int width = 0, height = 0;
+    float4* sky = nullptr;
+    int sky_w = 0, sky_h = 0;
+    float4* lut = nullptr;
+    int lut_n = 1;
+    float disc_params[6] = {0, 0, 0, 500.f, 30000.f, 1.f};
+    float T0 = 22800.0f, Tr0 = 2.0f;
+    int T_hold = 1;
+    bool T_from_M = true;   // reference law: r0 = 2M unless set explicitly
+    float* vol = nullptr;
+    int nr = 2, nphi = 1, nz = 2;
+    float vol_params[9] = {0, 0, 0, 0, 0, 0, 0, 0, 2.0f};
+    bool has_volume = false, vol_on = false;
+    float4* star_lut = nullptr;
+    int star_lut_n = 1;
+    float star_params[16] = {0};
+    float* out = nullptr;
+    float* rec = nullptr;
+    bool rec_on = false;
+    uint2* queue = nullptr;
+    size_t queue_cap = 0;
+    unsigned* qcount = nullptr;
+    unsigned long long* counters = nullptr;
+    unsigned* work_next = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool last_mixed = false;
+    bool have_last = false;
+    // batch
+    int nframes = 0;
+    std::vector<st::FrameRef> frames_host;
+    st::FrameRef* frames_dev = nullptr;
+    float* batch_out = nullptr;
+    size_t batch_out_n = 0;
+    int* tile_base_dev = nullptr;
+    int tile_base_tw = 0;
+    int total_items = 0;
+    float batch_st[8] = {0};
+    size_t batch_pixels = 0;
+    int tuned_tile = 0, tuned_waves = 0;
+    st::GraphCache graph;
+};
+
+namespace st {
+
+DeviceScene make_scene(const st_ctx* c);
+int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P);
 
 hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
                         const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s);
 hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
                           const FrameRef* frames_dev, int nblocks, hipStream_t s);
+hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
+                          int nblocks, hipStream_t s);
 hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s);
 hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,

@@ -73,24 +73,25 @@ __global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams 
     }
 }
 
-// FP64 re-trace of promoted rays.  BATCH: a queue entry's frame index
-// selects frames[]; otherwise every entry belongs to the single frame F0.
-template <bool KERR, bool VOL, bool BATCH>
-__global__ __launch_bounds__(256) void k_trace_queue64(DeviceScene D, TraceParams P, FrameRef F0,
-                                                       const FrameRef* frames) {
+// Re-trace of queued rays (FP64 for promoted rays, FP32 for the surrogate's
+// fallback).  BATCH: a queue entry's frame index selects frames[]; otherwise
+// every entry belongs to the single frame F0.
+template <typename T, bool KERR, bool VOL, bool BATCH>
+__global__ __launch_bounds__(256) void k_trace_queue(DeviceScene D, TraceParams P, FrameRef F0,
+                                                     const FrameRef* frames) {
     const unsigned n = *P.qcount;
     unsigned long long steps = 0;
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         const uint2 e = P.queue[q];
         const FrameRef& F = BATCH ? frames[e.x] : F0;
         const int i = (int)(e.y % (unsigned)F.width), j = (int)(e.y / (unsigned)F.width);
-        RayOut<double> o;
-        trace_ray<double, KERR, VOL, false>(D, P, F, i, j, o);
+        RayOut<T> o;
+        trace_ray<T, KERR, VOL, false>(D, P, F, i, j, o);
         store_ray(P, F, i, j, o);
         steps += (unsigned long long)o.steps;
     }
     const unsigned long long tot = wave_sum(steps);
-    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&P.counters[1], tot);
+    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 2], tot);
 }
 
 __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters, unsigned* work_next) {
@@ -198,7 +199,7 @@ hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceS
 hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
                           const FrameRef* frames_dev, int nblocks, hipStream_t s) {
 #define ST_Q(KK, VV, BB) \
-    hipLaunchKernelGGL((k_trace_queue64<KK, VV, BB>), dim3(nblocks), dim3(256), 0, s, D, P, F0, frames_dev)
+    hipLaunchKernelGGL((k_trace_queue<double, KK, VV, BB>), dim3(nblocks), dim3(256), 0, s, D, P, F0, frames_dev)
     const bool batch = frames_dev != nullptr;
     if (kerr) {
         if (vol) { if (batch) ST_Q(true, true, true); else ST_Q(true, true, false); }
@@ -208,6 +209,22 @@ hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const Trace
         else { if (batch) ST_Q(false, false, true); else ST_Q(false, false, false); }
     }
 #undef ST_Q
+    return hipGetLastError();
+}
+
+hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
+                          int nblocks, hipStream_t s) {
+    if (kerr) {
+        if (vol) hipLaunchKernelGGL((k_trace_queue<float, true, true, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
+                                    (const FrameRef*)nullptr);
+        else hipLaunchKernelGGL((k_trace_queue<float, true, false, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
+                                (const FrameRef*)nullptr);
+    } else {
+        if (vol) hipLaunchKernelGGL((k_trace_queue<float, false, true, false>), dim3(nblocks), dim3(256), 0, s, D, P,
+                                    F0, (const FrameRef*)nullptr);
+        else hipLaunchKernelGGL((k_trace_queue<float, false, false, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
+                                (const FrameRef*)nullptr);
+    }
     return hipGetLastError();
 }
 

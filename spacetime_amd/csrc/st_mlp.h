@@ -1,0 +1,82 @@
+// Surrogate lensing-map MLP: 5 -> 256 -> 256 -> 256 -> 256 -> 5 with SiLU,
+// BF16 weights/activations on gfx950 MFMA with FP32 accumulation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "st_trace.h"
+
+namespace st {
+namespace mlp {
+
+constexpr int NIN = 5;
+constexpr int HID = 256;
+constexpr int NOUT = 5;
+constexpr int NOUTP = 64;   // output layer padded to one GEMM tile
+
+// Offsets of the FP32 master parameters in one flat array.
+struct Layout {
+    static constexpr size_t W1 = 0;                               // [HID][NIN]
+    static constexpr size_t B1 = W1 + (size_t)HID * NIN;          // [HID]
+    static constexpr size_t W2 = B1 + HID;                        // [HID][HID]
+    static constexpr size_t B2 = W2 + (size_t)HID * HID;
+    static constexpr size_t W3 = B2 + HID;
+    static constexpr size_t B3 = W3 + (size_t)HID * HID;
+    static constexpr size_t W4 = B3 + HID;
+    static constexpr size_t B4 = W4 + (size_t)HID * HID;
+    static constexpr size_t W5 = B4 + HID;                        // [NOUTP][HID], rows >= NOUT zero
+    static constexpr size_t B5 = W5 + (size_t)NOUTP * HID;        // [NOUTP]
+    static constexpr size_t N = B5 + NOUTP;
+    __host__ __device__ static size_t W(int l) { return l == 1 ? W1 : l == 2 ? W2 : l == 3 ? W3 : l == 4 ? W4 : W5; }
+    __host__ __device__ static size_t B(int l) { return l == 1 ? B1 : l == 2 ? B2 : l == 3 ? B3 : l == 4 ? B4 : B5; }
+};
+
+// Input features from (r_cam, theta_cam, a, alpha, beta): centred and scaled
+// to O(1).  Shared by the training-set generator and inference.
+__host__ __device__ inline void features(float r, float th, float a, float al, float be, float x[NIN]) {
+    x[0] = (r - 27.0f) * (1.0f / 23.0f);
+    x[1] = (th - 1.5707963f) * (1.0f / 1.5707963f);
+    x[2] = (a - 0.495f) * (1.0f / 0.495f);
+    x[3] = al * (1.0f / 0.95f);
+    x[4] = be * (1.0f / 0.95f);
+}
+
+typedef __bf16 bf16;
+
+// One frame for surrogate inference: pinhole camera looking at the hole
+// (up +z) at KS radius r, polar angle th, spin a; axes = right, up, fwd.
+struct InferView {
+    int width, height;
+    float fov, ju, jv;
+    float r, th, a;
+    float axes[9];
+    float conf, delta_max;
+    int edge_px;
+};
+
+hipError_t launch_gemm_nt(bool atomic, const bf16* A, int lda, const bf16* Bt, int ldb, float* C, int ldc, int M,
+                          int N, int K, hipStream_t s);
+hipError_t launch_layer1(const float* X, const float* P, float* Z, bf16* H, bf16* Ht, int B, hipStream_t s);
+hipError_t launch_bias_silu(float* Z, const float* b, bf16* H, bf16* Ht, int B, hipStream_t s);
+hipError_t launch_loss(const float* Y, const float* b5, const float* T, bf16* dY, bf16* dYt, float* gb5, float* loss,
+                       int B, float w_dir, hipStream_t s);
+hipError_t launch_silu_back(const float* dH, const float* Z, bf16* dZ, bf16* dZt, float* gb, int B, float* dZf,
+                            hipStream_t s);
+hipError_t launch_w1_grad(const float* dZ1, const float* X, float* gW1, int B, hipStream_t s);
+hipError_t launch_adam(float* P, const float* G, float* Mm, float* Vv, size_t n, float lr, float bc1, float bc2,
+                       const float* sumsq, float clip, hipStream_t s);
+hipError_t launch_refresh(const float* P, bf16* W2b, bf16* W2t, bf16* W3b, bf16* W3t, bf16* W4b, bf16* W4t,
+                          bf16* W5b, bf16* W5t, hipStream_t s);
+hipError_t launch_make_samples(const DeviceScene& D, const TraceParams& P, const FrameRef* cams, const float* meta,
+                               int per_cam, int n, size_t total, float* X, float* T, int nblocks, hipStream_t s);
+hipError_t launch_fragment_weights(const float* P, bf16* W2f, bf16* W3f, bf16* W4f, bf16* W5f, hipStream_t s);
+hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const bf16* W4f, const bf16* W5f,
+                        const InferView& V, float* pred, int npix, hipStream_t s);
+hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene& D, float* out, float* rec,
+                         float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s);
+hipError_t launch_gather(const float* X, const float* T, size_t n, uint64_t seed, float* Xb, float* Tb, int B,
+                         hipStream_t s);
+
+}  // namespace mlp
+}  // namespace st

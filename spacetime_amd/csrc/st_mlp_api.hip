@@ -1,0 +1,355 @@
+// C ABI of the surrogate MLP: parameters, training set generation from the
+// RK4 kernel, training (forward, backward, Adam) and a gradient probe.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "spacetime_hip.h"
+#include "st_internal.h"
+#include "st_mlp.h"
+
+
+using namespace st::mlp;
+using st::fail;
+
+struct st_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* P = nullptr;     // FP32 master parameters
+    float* G = nullptr;     // gradients
+    float* Mm = nullptr;    // Adam moments
+    float* Vv = nullptr;
+    bf16 *W2b = nullptr, *W2t = nullptr, *W3b = nullptr, *W3t = nullptr, *W4b = nullptr, *W4t = nullptr;
+    bf16 *W5b = nullptr, *W5t = nullptr;
+    long long adam_t = 0;
+    // training set
+    float* X = nullptr;
+    float* T = nullptr;
+    size_t n_samples = 0;
+    // batch workspace
+    int bcap = 0;
+    float *Xb = nullptr, *Tb = nullptr, *Y = nullptr, *dH = nullptr, *dZ1f = nullptr;
+    float* Z[5] = {nullptr};
+    bf16 *H[5] = {nullptr}, *Ht[5] = {nullptr};
+    bf16 *dY = nullptr, *dYt = nullptr, *dZ = nullptr, *dZt = nullptr;
+    float* loss = nullptr;      // [steps][3]
+    int loss_cap = 0;
+    float* sumsq = nullptr;
+};
+
+namespace {
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int refresh(st_mlp* m) {
+    ST_HIP(launch_refresh(m->P, m->W2b, m->W2t, m->W3b, m->W3t, m->W4b, m->W4t, m->W5b, m->W5t, m->stream));
+    return ST_OK;
+}
+
+int ensure_batch(st_mlp* m, int B) {
+    if (m->bcap >= B) return ST_OK;
+    dfree(m->Xb); dfree(m->Tb); dfree(m->Y); dfree(m->dH); dfree(m->dZ1f);
+    for (int l = 1; l <= 4; ++l) { dfree(m->Z[l]); dfree(m->H[l]); dfree(m->Ht[l]); }
+    dfree(m->dY); dfree(m->dYt); dfree(m->dZ); dfree(m->dZt);
+    const size_t b = (size_t)B;
+    ST_HIP(hipMalloc(&m->Xb, b * NIN * sizeof(float)));
+    ST_HIP(hipMalloc(&m->Tb, b * 8 * sizeof(float)));
+    ST_HIP(hipMalloc(&m->Y, b * NOUTP * sizeof(float)));
+    ST_HIP(hipMalloc(&m->dH, b * HID * sizeof(float)));
+    ST_HIP(hipMalloc(&m->dZ1f, b * HID * sizeof(float)));
+    for (int l = 1; l <= 4; ++l) {
+        ST_HIP(hipMalloc(&m->Z[l], b * HID * sizeof(float)));
+        ST_HIP(hipMalloc(&m->H[l], b * HID * sizeof(bf16)));
+        ST_HIP(hipMalloc(&m->Ht[l], b * HID * sizeof(bf16)));
+    }
+    ST_HIP(hipMalloc(&m->dY, b * NOUTP * sizeof(bf16)));
+    ST_HIP(hipMalloc(&m->dYt, b * NOUTP * sizeof(bf16)));
+    ST_HIP(hipMalloc(&m->dZ, b * HID * sizeof(bf16)));
+    ST_HIP(hipMalloc(&m->dZt, b * HID * sizeof(bf16)));
+    m->bcap = B;
+    return ST_OK;
+}
+
+int gemm(st_mlp* m, const bf16* A, int lda, const bf16* Bt, int ldb, float* C, int ldc, int M, int N, int K,
+         bool atomic) {
+    if (M % 64 || N % 64 || K % 32) return fail(ST_ERR_INVALID_ARG, "gemm shape must be 64x64 tiles, K % 32");
+    ST_HIP(launch_gemm_nt(atomic, A, lda, Bt, ldb, C, ldc, M, N, K, m->stream));
+    return ST_OK;
+}
+
+// One forward + backward pass over the B samples in Xb/Tb; gradients are
+// accumulated into G (which the caller zeroes) and loss sums into loss3.
+int forward_backward(st_mlp* m, int B, float* loss3, float w_dir) {
+    int rc;
+    ST_HIP(launch_layer1(m->Xb, m->P, m->Z[1], m->H[1], m->Ht[1], B, m->stream));
+    const bf16* Wb[5] = {nullptr, nullptr, m->W2b, m->W3b, m->W4b};
+    const bf16* Wt[5] = {nullptr, nullptr, m->W2t, m->W3t, m->W4t};
+    for (int l = 2; l <= 4; ++l) {
+        if ((rc = gemm(m, m->H[l - 1], HID, Wb[l], HID, m->Z[l], HID, B, HID, HID, false)) != ST_OK) return rc;
+        ST_HIP(launch_bias_silu(m->Z[l], m->P + Layout::B(l), m->H[l], m->Ht[l], B, m->stream));
+    }
+    if ((rc = gemm(m, m->H[4], HID, m->W5b, HID, m->Y, NOUTP, B, NOUTP, HID, false)) != ST_OK) return rc;
+    ST_HIP(launch_loss(m->Y, m->P + Layout::B5, m->Tb, m->dY, m->dYt, m->G + Layout::B5, loss3, B, w_dir, m->stream));
+    // layer 5
+    if ((rc = gemm(m, m->dYt, B, m->Ht[4], B, m->G + Layout::W5, HID, NOUTP, HID, B, true)) != ST_OK) return rc;
+    if ((rc = gemm(m, m->dY, NOUTP, m->W5t, NOUTP, m->dH, HID, B, HID, NOUTP, false)) != ST_OK) return rc;
+    // layers 4..2
+    for (int l = 4; l >= 2; --l) {
+        ST_HIP(launch_silu_back(m->dH, m->Z[l], m->dZ, m->dZt, m->G + Layout::B(l), B, nullptr, m->stream));
+        if ((rc = gemm(m, m->dZt, B, m->Ht[l - 1], B, m->G + Layout::W(l), HID, HID, HID, B, true)) != ST_OK) return rc;
+        if ((rc = gemm(m, m->dZ, HID, Wt[l], HID, m->dH, HID, B, HID, HID, false)) != ST_OK) return rc;
+    }
+    // layer 1
+    ST_HIP(launch_silu_back(m->dH, m->Z[1], m->dZ, m->dZt, m->G + Layout::B1, B, m->dZ1f, m->stream));
+    ST_HIP(launch_w1_grad(m->dZ1f, m->Xb, m->G + Layout::W1, B, m->stream));
+    return ST_OK;
+}
+
+struct Rng {
+    uint64_t s[4];
+    static uint64_t sm(uint64_t& x) {
+        uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    explicit Rng(uint64_t seed) {
+        for (int i = 0; i < 4; ++i) s[i] = sm(seed);
+    }
+    static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace
+
+extern "C" {
+
+size_t st_mlp_param_count(void) { return Layout::N; }
+
+int st_mlp_create(st_mlp** out, uint64_t seed) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    st_mlp* m = new st_mlp();
+    ST_HIP(hipGetDevice(&m->device));
+    ST_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    const size_t n = Layout::N;
+    ST_HIP(hipMalloc(&m->P, n * sizeof(float)));
+    ST_HIP(hipMalloc(&m->G, n * sizeof(float)));
+    ST_HIP(hipMalloc(&m->Mm, n * sizeof(float)));
+    ST_HIP(hipMalloc(&m->Vv, n * sizeof(float)));
+    ST_HIP(hipMemset(m->Mm, 0, n * sizeof(float)));
+    ST_HIP(hipMemset(m->Vv, 0, n * sizeof(float)));
+    ST_HIP(hipMemset(m->G, 0, n * sizeof(float)));
+    ST_HIP(hipMalloc(&m->sumsq, 4 * sizeof(float)));
+    bf16** w[8] = {&m->W2b, &m->W2t, &m->W3b, &m->W3t, &m->W4b, &m->W4t, &m->W5b, &m->W5t};
+    for (int i = 0; i < 8; ++i) ST_HIP(hipMalloc(w[i], (size_t)HID * HID * sizeof(bf16)));
+    // He-uniform initialisation, zero biases, small output layer
+    std::vector<float> h(n, 0.0f);
+    Rng rng(seed);
+    auto fill = [&](size_t off, int rows, int cols, double gain) {
+        const double b = gain * std::sqrt(6.0 / cols);
+        for (int i = 0; i < rows * cols; ++i) h[off + i] = (float)((2.0 * rng.uniform() - 1.0) * b);
+    };
+    fill(Layout::W1, HID, NIN, 1.0);
+    fill(Layout::W2, HID, HID, 1.0);
+    fill(Layout::W3, HID, HID, 1.0);
+    fill(Layout::W4, HID, HID, 1.0);
+    fill(Layout::W5, NOUT, HID, 0.1);
+    ST_HIP(hipMemcpy(m->P, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    int rc = refresh(m);
+    if (rc != ST_OK) return rc;
+    ST_HIP(hipStreamSynchronize(m->stream));
+    *out = m;
+    return ST_OK;
+}
+
+int st_mlp_destroy(st_mlp* m) {
+    if (!m) return ST_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    dfree(m->P); dfree(m->G); dfree(m->Mm); dfree(m->Vv); dfree(m->sumsq); dfree(m->loss);
+    dfree(m->W2b); dfree(m->W2t); dfree(m->W3b); dfree(m->W3t); dfree(m->W4b); dfree(m->W4t);
+    dfree(m->W5b); dfree(m->W5t); dfree(m->X); dfree(m->T);
+    dfree(m->Xb); dfree(m->Tb); dfree(m->Y); dfree(m->dH); dfree(m->dZ1f);
+    for (int l = 1; l <= 4; ++l) { dfree(m->Z[l]); dfree(m->H[l]); dfree(m->Ht[l]); }
+    dfree(m->dY); dfree(m->dYt); dfree(m->dZ); dfree(m->dZt);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return ST_OK;
+}
+
+int st_mlp_get_params(st_mlp* m, float* host, size_t n) {
+    if (!m || !host || n != Layout::N) return fail(ST_ERR_INVALID_ARG, "bad parameter buffer");
+    ST_HIP(hipStreamSynchronize(m->stream));
+    ST_HIP(hipMemcpy(host, m->P, n * sizeof(float), hipMemcpyDeviceToHost));
+    return ST_OK;
+}
+
+int st_mlp_set_params(st_mlp* m, const float* host, size_t n) {
+    if (!m || !host || n != Layout::N) return fail(ST_ERR_INVALID_ARG, "bad parameter buffer");
+    ST_HIP(hipStreamSynchronize(m->stream));
+    ST_HIP(hipMemcpy(m->P, host, n * sizeof(float), hipMemcpyHostToDevice));
+    // padded output rows stay zero
+    std::vector<float> z((size_t)(NOUTP - NOUT) * HID, 0.0f);
+    ST_HIP(hipMemcpy(m->P + Layout::W5 + (size_t)NOUT * HID, z.data(), z.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+    ST_HIP(hipMemcpy(m->P + Layout::B5 + NOUT, z.data(), (NOUTP - NOUT) * sizeof(float), hipMemcpyHostToDevice));
+    m->adam_t = 0;
+    ST_HIP(hipMemset(m->Mm, 0, n * sizeof(float)));
+    ST_HIP(hipMemset(m->Vv, 0, n * sizeof(float)));
+    int rc = refresh(m);
+    if (rc != ST_OK) return rc;
+    ST_HIP(hipStreamSynchronize(m->stream));
+    return ST_OK;
+}
+
+int st_mlp_grad(st_mlp* m, const float* x, const float* t, int n, float* y_out, float* grad_out, float w_dir,
+                float* loss_out) {
+    if (!m || !x || !t || n <= 0 || n % 64) return fail(ST_ERR_INVALID_ARG, "n must be a positive multiple of 64");
+    int rc = ensure_batch(m, n);
+    if (rc != ST_OK) return rc;
+    float* l3 = nullptr;
+    ST_HIP(hipMalloc(&l3, 3 * sizeof(float)));
+    ST_HIP(hipMemsetAsync(l3, 0, 3 * sizeof(float), m->stream));
+    ST_HIP(hipMemcpyAsync(m->Xb, x, (size_t)n * NIN * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    ST_HIP(hipMemcpyAsync(m->Tb, t, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    ST_HIP(hipMemsetAsync(m->G, 0, Layout::N * sizeof(float), m->stream));
+    rc = forward_backward(m, n, l3, w_dir);
+    if (rc != ST_OK) { dfree(l3); return rc; }
+    ST_HIP(hipStreamSynchronize(m->stream));
+    if (y_out) {
+        std::vector<float> Y((size_t)n * NOUTP), b5(NOUTP);
+        ST_HIP(hipMemcpy(Y.data(), m->Y, Y.size() * sizeof(float), hipMemcpyDeviceToHost));
+        ST_HIP(hipMemcpy(b5.data(), m->P + Layout::B5, NOUTP * sizeof(float), hipMemcpyDeviceToHost));
+        for (int s = 0; s < n; ++s)
+            for (int k = 0; k < NOUT; ++k) y_out[(size_t)s * NOUT + k] = Y[(size_t)s * NOUTP + k] + b5[k];
+    }
+    if (grad_out) ST_HIP(hipMemcpy(grad_out, m->G, Layout::N * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss_out) ST_HIP(hipMemcpy(loss_out, l3, 3 * sizeof(float), hipMemcpyDeviceToHost));
+    dfree(l3);
+    return ST_OK;
+}
+
+int st_mlp_generate(st_mlp* m, st_ctx* c, const st_surrogate_domain* dom, const float sp[8],
+                    const st_trace_args* args) {
+    if (!m || !c || !dom || !sp || !args) return fail(ST_ERR_INVALID_ARG, "null argument");
+    const int n = dom->side;
+    const int ncam = dom->n_cameras;
+    if (n < 8 || ncam < 1 || !(dom->r_max > dom->r_min) || dom->a_max > 0.999f || dom->a_min < 0.0f)
+        return fail(ST_ERR_INVALID_ARG, "bad surrogate domain");
+    const int per = n * n;
+    const size_t total = (size_t)ncam * per;
+    dfree(m->X); dfree(m->T);
+    m->X = m->T = nullptr;
+    ST_HIP(hipMalloc(&m->X, total * NIN * sizeof(float)));
+    ST_HIP(hipMalloc(&m->T, total * 8 * sizeof(float)));
+    m->n_samples = total;
+    // cameras: r, theta, a drawn from the domain; static/ZAMO observer at
+    // (sqrt(r^2+a^2) sin th, 0, r cos th) looking at the hole, up +z
+    std::vector<st::FrameRef> fr(ncam);
+    std::vector<float> meta((size_t)ncam * 13);
+    Rng rng(dom->seed);
+    for (int k = 0; k < ncam; ++k) {
+        const double r = dom->r_min + (dom->r_max - dom->r_min) * rng.uniform();
+        const double th = 0.05 + (M_PI - 0.1) * rng.uniform();
+        const double a = dom->a_min + (dom->a_max - dom->a_min) * rng.uniform();
+        const double rho = std::sqrt(r * r + a * a);
+        const double pos[3] = {rho * std::sin(th), 0.0, r * std::cos(th)};
+        const double tgt[3] = {0.0, 0.0, 0.0}, up[3] = {0.0, 0.0, 1.0}, vel[3] = {0.0, 0.0, 0.0};
+        double cam[49];
+        int rc = st_camera_params(pos, tgt, up, dom->fov, vel, 1.0, a, 0.0, 1.0, 0, cam);
+        if (rc != ST_OK) return fail(rc, "camera parameters for the training set");
+        st::FrameRef& F = fr[k];
+        std::memset(&F, 0, sizeof(F));
+        for (int q = 0; q < 49; ++q) { F.c32[q] = (float)cam[q]; F.c64[q] = cam[q]; }
+        F.has64 = 1;
+        F.has_spin = 1;
+        F.spin = (float)a;
+        F.width = F.height = n;
+        // camera axes (world): fwd = -pos/|pos|, right = fwd x z, up = right x fwd
+        double f[3] = {-pos[0], -pos[1], -pos[2]};
+        const double fn = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        for (double& v : f) v /= fn;
+        double rr[3] = {f[1], -f[0], 0.0};
+        const double rn = std::sqrt(rr[0] * rr[0] + rr[1] * rr[1]);
+        for (double& v : rr) v /= rn;
+        const double uu[3] = {rr[1] * f[2] - rr[2] * f[1], rr[2] * f[0] - rr[0] * f[2], rr[0] * f[1] - rr[1] * f[0]};
+        float* mm = meta.data() + (size_t)k * 13;
+        mm[0] = (float)r; mm[1] = (float)th; mm[2] = (float)a;
+        for (int q = 0; q < 3; ++q) { mm[3 + q] = (float)rr[q]; mm[6 + q] = (float)uu[q]; mm[9 + q] = (float)f[q]; }
+        mm[12] = dom->fov;
+    }
+    st::FrameRef* fr_dev = nullptr;
+    float* meta_dev = nullptr;
+    ST_HIP(hipMalloc(&fr_dev, sizeof(st::FrameRef) * ncam));
+    ST_HIP(hipMalloc(&meta_dev, sizeof(float) * meta.size()));
+    ST_HIP(hipMemcpy(fr_dev, fr.data(), sizeof(st::FrameRef) * ncam, hipMemcpyHostToDevice));
+    ST_HIP(hipMemcpy(meta_dev, meta.data(), sizeof(float) * meta.size(), hipMemcpyHostToDevice));
+    st::TraceParams P;
+    int rc = st::fill_params(c, sp, args, P);
+    if (rc != ST_OK) { dfree(fr_dev); dfree(meta_dev); return rc; }
+    P.jw = 1.0f; P.ju = 0.0f; P.jv = 0.0f; P.jseed = (int)(dom->seed & 0x7fffffff);
+    st::DeviceScene D = st::make_scene(c);
+    D.disc_on = 0;
+    D.star_params[0] = 0.0f;
+    const int nblocks = c->num_cu * 8;
+    const hipError_t e = launch_make_samples(D, P, fr_dev, meta_dev, per, n, total, m->X, m->T, nblocks, m->stream);
+    const hipError_t e2 = hipStreamSynchronize(m->stream);
+    dfree(fr_dev);
+    dfree(meta_dev);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(ST_ERR_LAUNCH, std::string("sample generation: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    return ST_OK;
+}
+
+int st_mlp_train(st_mlp* m, const st_train_opts* o, float* loss_out) {
+    if (!m || !o) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!m->n_samples) return fail(ST_ERR_INVALID_ARG, "no training set: call st_mlp_generate first");
+    const int B = o->batch;
+    if (B <= 0 || B % 64) return fail(ST_ERR_INVALID_ARG, "batch must be a positive multiple of 64");
+    int rc = ensure_batch(m, B);
+    if (rc != ST_OK) return rc;
+    if (m->loss_cap < o->steps) {
+        dfree(m->loss);
+        ST_HIP(hipMalloc(&m->loss, sizeof(float) * 3 * (size_t)std::max(o->steps, 1)));
+        m->loss_cap = o->steps;
+    }
+    ST_HIP(hipMemsetAsync(m->loss, 0, sizeof(float) * 3 * (size_t)o->steps, m->stream));
+    const size_t n = Layout::N;
+    for (int step = 0; step < o->steps; ++step) {
+        const double prog = o->steps > 1 ? (double)step / (o->steps - 1) : 1.0;
+        const double warm = o->warmup > 0 ? std::min(1.0, (step + 1.0) / o->warmup) : 1.0;
+        const float lr = (float)(warm * (o->lr_final + 0.5 * (o->lr - o->lr_final) * (1.0 + std::cos(M_PI * prog))));
+        ST_HIP(launch_gather(m->X, m->T, m->n_samples, (uint64_t)(o->seed * 1000003ull + (uint64_t)m->adam_t), m->Xb,
+                             m->Tb, B, m->stream));
+        ST_HIP(hipMemsetAsync(m->G, 0, n * sizeof(float), m->stream));
+        rc = forward_backward(m, B, m->loss + 3 * (size_t)step, o->w_dir);
+        if (rc != ST_OK) return rc;
+        ST_HIP(hipMemsetAsync(m->sumsq, 0, sizeof(float), m->stream));
+        m->adam_t += 1;
+        const float bc1 = 1.0f - (float)std::pow(0.9, (double)m->adam_t);
+        const float bc2 = 1.0f - (float)std::pow(0.999, (double)m->adam_t);
+        ST_HIP(launch_adam(m->P, m->G, m->Mm, m->Vv, n, lr, bc1, bc2, m->sumsq, o->clip, m->stream));
+        rc = refresh(m);
+        if (rc != ST_OK) return rc;
+    }
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipStreamSynchronize(m->stream));
+    if (loss_out)
+        ST_HIP(hipMemcpy(loss_out, m->loss, sizeof(float) * 3 * (size_t)o->steps, hipMemcpyDeviceToHost));
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// Training and inference kernels of the surrogate MLP (st_mlp.h) for gfx950.
+//
+// GEMMs use v_mfma_f32_16x16x32_bf16 (BF16 inputs, FP32 accumulate).
+// Operand layouts of one 16x16x32 MFMA on a 64-lane wave (lane l):
+//   A (16 x 32, row-major source): row l%16, k = 8*(l/16) .. +7  (one 16-byte load)
+//   B (32 x 16) from B^T (16 x 32, row-major): column l%16, same k range
+//   C (16 x 16 FP32): 4 values, rows 4*(l/16)+v, column l%16
+// so every GEMM here is "NT": C[M][N] = sum_k A[M][k] * Bt[N][k], with both
+// operands contiguous along k.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "st_mlp.h"
+#include "st_trace.h"
+
+namespace st {
+namespace mlp {
+
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// C[M][N] (+)= A[M][K] * Bt[N][K]^T.  Block tile 64x64 (4 waves of 32x32,
+// each 2x2 MFMA tiles); grid (N/64, M/64, splits); split s sums k in
+// [s*kc, (s+1)*kc).  M, N multiples of 64; K and kc multiples of 32.
+// ATOMIC: accumulate into C with atomics (split-K weight gradients).
+template <bool ATOMIC>
+__global__ __launch_bounds__(256) void k_gemm_nt(const bf16* __restrict__ A, int lda, const bf16* __restrict__ Bt,
+                                                 int ldb, float* __restrict__ C, int ldc, int K, int kc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int kb = blockIdx.z * kc;
+    const int ke = min(K, kb + kc);
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16* a0 = A + (size_t)(m0 + r) * lda + kq;
+    const bf16* a1 = A + (size_t)(m0 + 16 + r) * lda + kq;
+    const bf16* b0 = Bt + (size_t)(n0 + r) * ldb + kq;
+    const bf16* b1 = Bt + (size_t)(n0 + 16 + r) * ldb + kq;
+    for (int k = kb; k < ke; k += 32) {
+        const bf16x8 va0 = *reinterpret_cast<const bf16x8*>(a0 + k);
+        const bf16x8 va1 = *reinterpret_cast<const bf16x8*>(a1 + k);
+        const bf16x8 vb0 = *reinterpret_cast<const bf16x8*>(b0 + k);
+        const bf16x8 vb1 = *reinterpret_cast<const bf16x8*>(b1 + k);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+    }
+    const int cr = (lane >> 4) * 4, cc = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float* p = C + (size_t)(m0 + 16 * i + cr + v) * ldc + n0 + 16 * j + cc;
+                if (ATOMIC) atomicAdd(p, acc[i][j][v]);
+                else *p = acc[i][j][v];
+            }
+}
+
+// Layer 1 (K = 5) on the vector ALU: Z = X W1^T + b1, H = silu(Z) in
+// row-major and transposed BF16.  Block: 64 rows x 64 columns.
+__global__ __launch_bounds__(256) void k_layer1(const float* __restrict__ X, const float* __restrict__ P,
+                                                float* __restrict__ Z, bf16* __restrict__ H, bf16* __restrict__ Ht,
+                                                int B) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const float* W1 = P + Layout::W1;
+    const float* b1 = P + Layout::B1;
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int rr = t / 64, cc = t % 64;
+        const int row = r0 + rr, col = c0 + cc;
+        const float* x = X + (size_t)row * NIN;
+        float z = b1[col];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) z += W1[col * NIN + k] * x[k];
+        Z[(size_t)row * HID + col] = z;
+        const float h = silu(z);
+        H[(size_t)row * HID + col] = (bf16)h;
+        tile[rr][cc] = h;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int cc = t / 64, rr = t % 64;
+        Ht[(size_t)(c0 + cc) * B + r0 + rr] = (bf16)tile[rr][cc];
+    }
+}
+
+// Hidden-layer epilogue: Z += b; H = silu(Z) (row-major and transposed BF16).
+__global__ __launch_bounds__(256) void k_bias_silu(float* __restrict__ Z, const float* __restrict__ b,
+                                                   bf16* __restrict__ H, bf16* __restrict__ Ht, int B) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int rr = t / 64, cc = t % 64;
+        const size_t idx = (size_t)(r0 + rr) * HID + c0 + cc;
+        const float z = Z[idx] + b[c0 + cc];
+        Z[idx] = z;
+        const float h = silu(z);
+        H[idx] = (bf16)h;
+        tile[rr][cc] = h;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int cc = t / 64, rr = t % 64;
+        Ht[(size_t)(c0 + cc) * B + r0 + rr] = (bf16)tile[rr][cc];
+    }
+}
+
+// Loss of one batch and its gradient w.r.t. the (padded) outputs Y[B][64]:
+//   direction: w_dir * |Y[0:3] + n - d|^2 / 3 on escaped rays (the net predicts
+//              the correction to the undeflected direction n)
+//   fate:      softmax cross-entropy of Y[3:5] against {escaped, captured}
+// T[s] = (d_x, d_y, d_z, fate, valid, n_x, n_y, n_z) in camera axes.
+// Writes dY (BF16 row-major and transposed, zero padding), adds the bias
+// gradient of layer 5 and the loss sums (dir, fate, count) with atomics.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, const float* __restrict__ b5,
+                                              const float* __restrict__ T, bf16* __restrict__ dY,
+                                              bf16* __restrict__ dYt, float* __restrict__ gb5,
+                                              float* __restrict__ loss, int B, float w_dir) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    float g[NOUT] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float ldir = 0.f, lfate = 0.f, cnt = 0.f;
+    if (s < B) {
+        const float* y = Y + (size_t)s * NOUTP;
+        const float* t = T + (size_t)s * 8;
+        float o[NOUT];
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) o[k] = y[k] + b5[k];
+        const float valid = t[4];
+        const float invB = 1.0f / (float)B;
+        if (valid > 0.f) {
+            cnt = 1.f;
+            const int fate = (int)t[3];
+            if (fate == 0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float e = o[k] + t[5 + k] - t[k];
+                    ldir += e * e * (1.0f / 3.0f);
+                    g[k] = w_dir * 2.0f * e * (1.0f / 3.0f) * invB;
+                }
+            }
+            const float m = fmaxf(o[3], o[4]);
+            const float e0 = __expf(o[3] - m), e1 = __expf(o[4] - m);
+            const float p0 = e0 / (e0 + e1), p1 = e1 / (e0 + e1);
+            lfate = -__logf(fmaxf(fate == 0 ? p0 : p1, 1e-30f));
+            g[3] = (p0 - (fate == 0 ? 1.f : 0.f)) * invB;
+            g[4] = (p1 - (fate == 0 ? 0.f : 1.f)) * invB;
+        }
+        bf16* d = dY + (size_t)s * NOUTP;
+        for (int k = 0; k < NOUTP; ++k) d[k] = (bf16)(k < NOUT ? g[k] : 0.f);
+        for (int k = 0; k < NOUTP; ++k) dYt[(size_t)k * B + s] = (bf16)(k < NOUT ? g[k] : 0.f);
+    }
+    // block reductions of the bias gradient and the loss sums
+    __shared__ float red[8][256];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) red[k][threadIdx.x] = g[k];
+    red[5][threadIdx.x] = ldir * w_dir;
+    red[6][threadIdx.x] = lfate;
+    red[7][threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 8; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NOUT; ++k) atomicAdd(&gb5[k], red[k][0]);
+        atomicAdd(&loss[0], red[5][0]);
+        atomicAdd(&loss[1], red[6][0]);
+        atomicAdd(&loss[2], red[7][0]);
+    }
+}
+
+// dZ = dH * silu'(Z) -> BF16 row-major + transposed; bias gradient += colsum.
+__global__ __launch_bounds__(256) void k_silu_back(const float* __restrict__ dH, const float* __restrict__ Z,
+                                                   bf16* __restrict__ dZ, bf16* __restrict__ dZt,
+                                                   float* __restrict__ gb, int B, float* __restrict__ dZf) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int rr = t / 64, cc = t % 64;
+        const size_t idx = (size_t)(r0 + rr) * HID + c0 + cc;
+        const float d = dH[idx] * silu_grad(Z[idx]);
+        dZ[idx] = (bf16)d;
+        if (dZf) dZf[idx] = d;
+        tile[rr][cc] = d;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+        const int cc = t / 64, rr = t % 64;
+        dZt[(size_t)(c0 + cc) * B + r0 + rr] = (bf16)tile[rr][cc];
+    }
+    __syncthreads();
+    // column sums for the bias gradient: 4 threads per column
+    if (threadIdx.x < 256) {
+        const int cc = threadIdx.x & 63, part = threadIdx.x >> 6;
+        float sacc = 0.f;
+        for (int rr = part * 16; rr < part * 16 + 16; ++rr) sacc += tile[rr][cc];
+        atomicAdd(&gb[c0 + cc], sacc);
+    }
+}
+
+// Layer-1 weight gradient: gW1[n][k] += sum_s dZ1[s][n] X[s][k] (FP32 dZ1).
+__global__ __launch_bounds__(256) void k_w1_grad(const float* __restrict__ dZ1, const float* __restrict__ X,
+                                                 float* __restrict__ gW1, int B) {
+    const int n = blockIdx.x;   // one block per hidden unit
+    float acc[NIN] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int s = threadIdx.x; s < B; s += blockDim.x) {
+        const float d = dZ1[(size_t)s * HID + n];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) acc[k] += d * X[(size_t)s * NIN + k];
+    }
+    __shared__ float red[NIN][256];
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < NIN; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < NIN) gW1[n * NIN + threadIdx.x] += red[threadIdx.x][0];
+}
+
+// Adam on the flat FP32 parameters (rows >= NOUT of layer 5 stay zero).
+// The gradient is scaled to global norm <= clip (clip <= 0: no clipping).
+__global__ void k_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ Mm,
+                       float* __restrict__ Vv, size_t n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2, const float* __restrict__ sumsq, float clip) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float norm = sqrtf(*sumsq);
+    const float clip_scale = (clip > 0.f && norm > clip) ? clip / norm : 1.0f;
+    const float g = G[i] * clip_scale;
+    const float m = b1 * Mm[i] + (1.f - b1) * g;
+    const float v = b2 * Vv[i] + (1.f - b2) * g * g;
+    Mm[i] = m;
+    Vv[i] = v;
+    P[i] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+}
+
+// Sum of squares of G (global gradient norm for clipping).
+__global__ void k_sumsq(const float* __restrict__ G, size_t n, float* __restrict__ out) {
+    float s = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += G[i] * G[i];
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// BF16 weight copies for the GEMMs: Wb[l] = W_l (row-major [out][in]) and
+// Wt[l] = W_l^T ([in][out]); layer 5 keeps its padded rows.
+__global__ void k_refresh_bf16(const float* __restrict__ P, bf16* __restrict__ W2b, bf16* __restrict__ W2t,
+                               bf16* __restrict__ W3b, bf16* __restrict__ W3t, bf16* __restrict__ W4b,
+                               bf16* __restrict__ W4t, bf16* __restrict__ W5b, bf16* __restrict__ W5t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < HID * HID) {
+        const int o = i / HID, k = i % HID;
+        const float w2 = P[Layout::W2 + i], w3 = P[Layout::W3 + i], w4 = P[Layout::W4 + i];
+        W2b[i] = (bf16)w2; W2t[(size_t)k * HID + o] = (bf16)w2;
+        W3b[i] = (bf16)w3; W3t[(size_t)k * HID + o] = (bf16)w3;
+        W4b[i] = (bf16)w4; W4t[(size_t)k * HID + o] = (bf16)w4;
+    }
+    if (i < NOUTP * HID) {
+        const int o = i / HID, k = i % HID;
+        const float w5 = o < NOUT ? P[Layout::W5 + i] : 0.f;
+        W5b[i] = (bf16)w5;
+        W5t[(size_t)k * NOUTP + o] = (bf16)w5;
+    }
+}
+
+// Training set from the RK4 kernel: sample s belongs to camera s / per_cam and
+// takes sensor point (i, j) of an n x n grid with hashed jitter; the camera
+// sees directions with tan(alpha), tan(beta) in [-fov, fov].
+// X[s] = features(r, theta, a, alpha, beta); T[s] = (d_cam, fate, valid, n_cam).
+__global__ __launch_bounds__(256) void k_make_samples(DeviceScene D, TraceParams P, const FrameRef* __restrict__ cams,
+                                                      const float* __restrict__ meta, int per_cam, int n,
+                                                      size_t total, float* __restrict__ X, float* __restrict__ T) {
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(s / per_cam), k = (int)(s % per_cam);
+        const int i = k % n, j = k / n;
+        const FrameRef& F = cams[c];
+        const float* m = meta + (size_t)c * 13;   // r, theta, a, right(3), up(3), fwd(3), fov
+        RayOut<float> o;
+        trace_ray<float, true, false, false>(D, P, F, i, j, o);
+        // the same sensor point trace_ray used
+        float ju = P.ju + sim_hash(i + 1, j + 1, P.jseed) * P.jw;
+        float jv = P.jv + sim_hash(i + 1, j + 1, P.jseed + 271) * P.jw;
+        const float half = 0.5f * (float)n;
+        const float u = ((float)i + ju - half) / half, v = ((float)j + jv - half) / half;
+        const float fov = m[12];
+        const float tu = u * fov, tv = v * fov;
+        const float inu = 1.0f / sqrtf(tu * tu + tv * tv + 1.0f);
+        features(m[0], m[1], m[2], atanf(tu), atanf(tv), X + s * NIN);
+        float* t = T + s * 8;
+        const float dx = o.dir[0], dy = o.dir[1], dz = o.dir[2];
+        t[0] = dx * m[3] + dy * m[4] + dz * m[5];
+        t[1] = dx * m[6] + dy * m[7] + dz * m[8];
+        t[2] = dx * m[9] + dy * m[10] + dz * m[11];
+        t[3] = (float)(o.fate == 0 ? 0 : 1);
+        t[4] = o.fate == 2 ? 0.f : 1.f;
+        t[5] = tu * inu;
+        t[6] = tv * inu;
+        t[7] = inu;
+    }
+}
+
+// Random mini-batch (with replacement) from the training set.
+__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ T, size_t n, uint64_t seed,
+                         float* __restrict__ Xb, float* __restrict__ Tb, int B) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(s + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    const size_t idx = (size_t)(z % (uint64_t)n);
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) Xb[(size_t)s * NIN + k] = X[idx * NIN + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Tb[(size_t)s * 8 + k] = T[idx * 8 + k];
+}
+
+// Inference weights in MFMA fragment order: for k-step ks (32 inputs) and
+// output tile nt (16 outputs), lane l's 8 BF16 B-operand values are the
+// contiguous Wf[((ks * ntiles + nt) * 64 + l) * 8 .. +7] = W[nt*16 + l%16][ks*32 + 8*(l/16) ..],
+// so one wave's fragment is a single 1 KiB coalesced load.
+__global__ void k_fragment_weights(const float* __restrict__ P, bf16* __restrict__ W2f, bf16* __restrict__ W3f,
+                                   bf16* __restrict__ W4f, bf16* __restrict__ W5f) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // one per (ks, nt, lane, e)
+    const int e = idx & 7, lane = (idx >> 3) & 63, rest = idx >> 9;
+    if (rest < 8 * 16) {
+        const int nt = rest % 16, ks = rest / 16;
+        const int o = nt * 16 + (lane & 15), k = ks * 32 + 8 * (lane >> 4) + e;
+        W2f[idx] = (bf16)P[Layout::W2 + (size_t)o * HID + k];
+        W3f[idx] = (bf16)P[Layout::W3 + (size_t)o * HID + k];
+        W4f[idx] = (bf16)P[Layout::W4 + (size_t)o * HID + k];
+    }
+    if (rest < 8) {
+        const int ks = rest;
+        const int o = lane & 15, k = ks * 32 + 8 * (lane >> 4) + e;
+        W5f[idx] = (bf16)(o < NOUT ? P[Layout::W5 + (size_t)o * HID + k] : 0.f);
+    }
+}
+
+__device__ __forceinline__ float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+
+// Fused inference over the pixels of one frame: 128 rays per 256-thread
+// block; activations stay in LDS as BF16 (row stride 264 keeps the 16-byte
+// A-fragment reads conflict-free), each wave computes 64 of the 256 outputs
+// of a layer for all 128 rays with 8 x 4 16x16x32 MFMA tiles.
+// pred[p] = (d_world xyz, p_escape, |correction|, unused).
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, const bf16* __restrict__ W2f,
+                                                   const bf16* __restrict__ W3f, const bf16* __restrict__ W4f,
+                                                   const bf16* __restrict__ W5f, InferView V,
+                                                   float* __restrict__ pred, int npix) {
+    constexpr int ROWS = 128, LD = HID + 8;
+    __shared__ bf16 Hs[ROWS * LD];
+    __shared__ float Xs[ROWS][NIN + 3];
+    __shared__ float Ys[ROWS][8];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int base = blockIdx.x * ROWS;
+    if (tid < ROWS) {
+        const int p = min(base + tid, npix - 1);
+        const int i = p % V.width, j = p / V.width;
+        const float half = 0.5f * (float)V.height;
+        const float u = ((float)i + V.ju - 0.5f * (float)V.width) / half;
+        const float v = ((float)j + V.jv - 0.5f * (float)V.height) / half;
+        const float tu = u * V.fov, tv = v * V.fov;
+        float x[NIN];
+        features(V.r, V.th, V.a, atanf(tu), atanf(tv), x);
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) Xs[tid][k] = x[k];
+        Xs[tid][5] = tu;
+        Xs[tid][6] = tv;
+    }
+    __syncthreads();
+    {   // layer 1 on the vector ALU: thread tid owns hidden unit tid
+        float w[NIN];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) w[k] = P[Layout::W1 + tid * NIN + k];
+        const float b = P[Layout::B1 + tid];
+        for (int r = 0; r < ROWS; ++r) {
+            float z = b;
+#pragma unroll
+            for (int k = 0; k < NIN; ++k) z += w[k] * Xs[r][k];
+            Hs[r * LD + tid] = (bf16)silu_fast(z);
+        }
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+    const bf16* Wl[3] = {W2f, W3f, W4f};
+#pragma unroll 1
+    for (int l = 0; l < 3; ++l) {
+        const bf16* Wf = Wl[l];
+        f32x4 acc[8][4];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) acc[i][jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int ks = 0; ks < 8; ++ks) {
+            bf16x8 bq[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+                bq[jj] = *reinterpret_cast<const bf16x8*>(Wf + (((size_t)ks * 16 + wave * 4 + jj) * 64 + lane) * 8);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const bf16x8 aq = *reinterpret_cast<const bf16x8*>(&Hs[(i * 16 + lr) * LD + ks * 32 + lk]);
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                    acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq, bq[jj], acc[i][jj], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        const float* bias = P + Layout::B(l + 2);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int col = wave * 64 + jj * 16 + lr;
+            const float bb = bias[col];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    Hs[(i * 16 + 4 * (lane >> 4) + v) * LD + col] = (bf16)silu_fast(acc[i][jj][v] + bb);
+        }
+        __syncthreads();
+    }
+    {   // output layer: wave w takes row tiles 2w, 2w+1 of the single 16-wide tile
+        f32x4 acc5[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const bf16x8 bq = *reinterpret_cast<const bf16x8*>(W5f + ((size_t)ks * 64 + lane) * 8);
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                const bf16x8 aq =
+                    *reinterpret_cast<const bf16x8*>(&Hs[((2 * wave + ii) * 16 + lr) * LD + ks * 32 + lk]);
+                acc5[ii] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq, bq, acc5[ii], 0, 0, 0);
+            }
+        }
+        if (lr < NOUT) {
+            const float bb = P[Layout::B5 + lr];
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Ys[(2 * wave + ii) * 16 + 4 * (lane >> 4) + v][lr] = acc5[ii][v] + bb;
+        }
+    }
+    __syncthreads();
+    if (tid < ROWS && base + tid < npix) {
+        const float tu = Xs[tid][5], tv = Xs[tid][6];
+        const float inu = 1.0f / sqrtf(tu * tu + tv * tv + 1.0f);
+        const float dx = Ys[tid][0], dy = Ys[tid][1], dz = Ys[tid][2];
+        float cx = tu * inu + dx, cy = tv * inu + dy, cz = inu + dz;
+        const float cl = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
+        cx *= cl; cy *= cl; cz *= cl;
+        const float m = fmaxf(Ys[tid][3], Ys[tid][4]);
+        const float e0 = __expf(Ys[tid][3] - m), e1 = __expf(Ys[tid][4] - m);
+        float* o = pred + (size_t)(base + tid) * 8;
+        o[0] = cx * V.axes[0] + cy * V.axes[3] + cz * V.axes[6];
+        o[1] = cx * V.axes[1] + cy * V.axes[4] + cz * V.axes[7];
+        o[2] = cx * V.axes[2] + cy * V.axes[5] + cz * V.axes[8];
+        o[3] = e0 / (e0 + e1);
+        o[4] = sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+}
+
+// Surrogate shading + fallback selection.  A pixel falls back to RK4 when its
+// fate confidence is below conf, when a pixel within edge_px has the other
+// predicted fate (shadow edge), or when the predicted correction exceeds
+// delta_max (strong lensing near the photon ring).  Fallback pixels are
+// appended to the queue (one atomic per wave); the rest are shaded here.
+__global__ __launch_bounds__(256) void k_surrogate_select(const float* __restrict__ pred, InferView V, DeviceScene D,
+                                                          float* __restrict__ out, float* __restrict__ rec,
+                                                          float weight, uint2* __restrict__ queue,
+                                                          unsigned* __restrict__ qcount, int npix) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool push = false;
+    if (p < npix) {
+        const int i = p % V.width, j = p / V.width;
+        const float* q = pred + (size_t)p * 8;
+        const float pe = q[3];
+        const bool esc = pe >= 0.5f;
+        push = fmaxf(pe, 1.0f - pe) < V.conf || q[4] > V.delta_max;
+        for (int dj = -V.edge_px; dj <= V.edge_px && !push; ++dj) {
+            const int jj = j + dj;
+            if (jj < 0 || jj >= V.height) continue;
+            for (int di = -V.edge_px; di <= V.edge_px; ++di) {
+                const int ii = i + di;
+                if (ii < 0 || ii >= V.width) continue;
+                if ((pred[((size_t)jj * V.width + ii) * 8 + 3] >= 0.5f) != esc) { push = true; break; }
+            }
+        }
+        if (!push) {
+            float col[3] = {0.f, 0.f, 0.f};
+            if (esc) {
+                const float th = acosf(fminf(fmaxf(q[2], -1.0f), 1.0f));
+                const float ph = atan2f(q[1], q[0]);
+                sample_sky<float>(D, th, ph, col);
+            }
+            float* o = out + (size_t)p * 3;
+            o[0] += weight * col[0]; o[1] += weight * col[1]; o[2] += weight * col[2];
+            if (rec) {
+                float* r = rec + (size_t)p * REC_N;
+                r[0] = esc ? 0.f : 1.f; r[1] = 0.f;
+                r[2] = esc ? q[0] : 0.f; r[3] = esc ? q[1] : 0.f; r[4] = esc ? q[2] : 0.f;
+                r[5] = col[0]; r[6] = col[1]; r[7] = col[2];
+                r[8] = 0.f; r[9] = 0.f; r[10] = 8.f; r[11] = 0.f;
+            }
+        }
+    }
+    const unsigned long long mask = __ballot(push);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (push) queue[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = make_uint2(0u, (unsigned)p);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_gemm_nt(bool atomic, const bf16* A, int lda, const bf16* Bt, int ldb, float* C, int ldc, int M,
+                          int N, int K, hipStream_t s) {
+    if (!atomic) {
+        hipLaunchKernelGGL((k_gemm_nt<false>), dim3(N / 64, M / 64, 1), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc, K, K);
+    } else {
+        const int kc = K < 512 ? K : 512;
+        const int splits = (K + kc - 1) / kc;
+        hipLaunchKernelGGL((k_gemm_nt<true>), dim3(N / 64, M / 64, splits), dim3(256), 0, s, A, lda, Bt, ldb, C, ldc,
+                           K, kc);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_layer1(const float* X, const float* P, float* Z, bf16* H, bf16* Ht, int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_layer1, dim3(HID / 64, B / 64), dim3(256), 0, s, X, P, Z, H, Ht, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_silu(float* Z, const float* b, bf16* H, bf16* Ht, int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_bias_silu, dim3(HID / 64, B / 64), dim3(256), 0, s, Z, b, H, Ht, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss(const float* Y, const float* b5, const float* T, bf16* dY, bf16* dYt, float* gb5, float* loss,
+                       int B, float w_dir, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss, dim3((B + 255) / 256), dim3(256), 0, s, Y, b5, T, dY, dYt, gb5, loss, B, w_dir);
+    return hipGetLastError();
+}
+
+hipError_t launch_silu_back(const float* dH, const float* Z, bf16* dZ, bf16* dZt, float* gb, int B, float* dZf,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_silu_back, dim3(HID / 64, B / 64), dim3(256), 0, s, dH, Z, dZ, dZt, gb, B, dZf);
+    return hipGetLastError();
+}
+
+hipError_t launch_w1_grad(const float* dZ1, const float* X, float* gW1, int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_w1_grad, dim3(HID), dim3(256), 0, s, dZ1, X, gW1, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* P, const float* G, float* Mm, float* Vv, size_t n, float lr, float bc1, float bc2,
+                       const float* sumsq, float clip, hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, s, G, n, const_cast<float*>(sumsq));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, G, Mm, Vv, n, lr, 0.9f, 0.999f,
+                       1e-8f, bc1, bc2, sumsq, clip);
+    return hipGetLastError();
+}
+
+hipError_t launch_refresh(const float* P, bf16* W2b, bf16* W2t, bf16* W3b, bf16* W3t, bf16* W4b, bf16* W4t,
+                          bf16* W5b, bf16* W5t, hipStream_t s) {
+    hipLaunchKernelGGL(k_refresh_bf16, dim3((HID * HID + 255) / 256), dim3(256), 0, s, P, W2b, W2t, W3b, W3t, W4b,
+                       W4t, W5b, W5t);
+    return hipGetLastError();
+}
+
+hipError_t launch_make_samples(const DeviceScene& D, const TraceParams& P, const FrameRef* cams, const float* meta,
+                               int per_cam, int n, size_t total, float* X, float* T, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_make_samples, dim3(nblocks), dim3(256), 0, s, D, P, cams, meta, per_cam, n, total, X, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_fragment_weights(const float* P, bf16* W2f, bf16* W3f, bf16* W4f, bf16* W5f, hipStream_t s) {
+    hipLaunchKernelGGL(k_fragment_weights, dim3(HID * HID / 256), dim3(256), 0, s, P, W2f, W3f, W4f, W5f);
+    return hipGetLastError();
+}
+
+hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const bf16* W4f, const bf16* W5f,
+                        const InferView& V, float* pred, int npix, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3((npix + 127) / 128), dim3(256), 0, s, P, W2f, W3f, W4f, W5f, V, pred, npix);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene& D, float* out, float* rec,
+                         float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s) {
+    hipLaunchKernelGGL(k_surrogate_select, dim3((npix + 255) / 256), dim3(256), 0, s, pred, V, D, out, rec, weight,
+                       queue, qcount, npix);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const float* X, const float* T, size_t n, uint64_t seed, float* Xb, float* Tb, int B,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, s, X, T, n, seed, Xb, Tb, B);
+    return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace st

@@ -56,6 +56,8 @@ struct FrameRef {
     float c32[49];           // cam_params[49]
     double c64[49];          // FP64 copy of the same block
     int has64;               // c64 valid
+    int has_spin;            // spin below overrides the launch's a (per-camera spin)
+    float spin;
     int width, height;
     float* out;              // frame output base
     float* rec;              // frame record base (may be null)
@@ -378,7 +380,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
                                           RayOut<T>& o) {
     using A = Ar<T>;
     const int I = i + 1, J = j + 1;   // the reference hashes 1-based pixel indices
-    const T M = T(P.M), a = KERR ? T(P.a) : T(0);
+    const T M = T(P.M), a = KERR ? T(F.has_spin ? F.spin : P.a) : T(0);
     o.routed = false;
     o.flags = (sizeof(T) == 8) ? 1 : 0;
 
@@ -475,15 +477,20 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     K.scam = P.relativistic ? T(1) / clampv(A::fabs(pt), T(0.05), T(20)) : T(1);
     isco(M, a, K.r_isco, K.E_isco, K.L_isco);
 
+    // A ray whose Killing energy E = p_t (the backward vector's covariant
+    // t component) is not positive cannot have come from infinity: from a
+    // camera inside the horizon it leads to the other exterior.  Capture
+    // mode 1 paints it black without integrating.
+    const bool no_sky = P.capture_mode == 1 && !(pt > T(0));
     // ---- launch-time radial potential (src/metal.jl:1008-1094) ----
     bool captured0 = false;
     const T rplus = M + A::psqrt(mx(M * M - a * a, T(0)));
-    const bool need_bardeen = (KERR && P.bardeen && P.a != 0.0f) || ROUTE;
+    const bool need_bardeen = !no_sky && ((KERR && P.bardeen && a != T(0)) || ROUTE);
     if (need_bardeen) {
         bool cap, graze, ing;
         T rc, Rc, Rpp;
         bardeen<T, KERR>(s, pt, M, a, cap, graze, ing, rc, Rc, Rpp);
-        if (KERR && P.bardeen && P.a != 0.0f) captured0 = cap && ing;
+        if (KERR && P.bardeen && a != T(0)) captured0 = cap && ing;
         if (cap && ing) o.flags |= 2;
         if (graze) o.flags |= 4;
         if constexpr (ROUTE) {
@@ -507,8 +514,14 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     const T ds_gas = T(mstep) * T(P.dt);
     const T r_escape = T(P.r_escape), dt = T(P.dt), h_coef = T(P.h_coef), h_floor = T(P.h_floor);
     const T h_cap = T(P.h_cap), rkill2 = T(P.rkill2);
+    // Capture mode 1: a ray below the capture radius and moving inward is
+    // captured (below the prograde photon orbit an inward backward ray never
+    // turns out again).  From a camera inside the horizon backward rays first
+    // move outward; they are captured only if they turn back.
     const T rcap = T(P.capture_factor) * rplus;
     const T rcap2 = rcap * rcap;
+    const T rfloor2 = T(0.09) * M * M;
+    T rk2_prev = ks_r2<T, KERR>(s.x, s.y, s.z, a) > rcap2 ? T(1e30) : T(0);
     const T rprom = T(P.promote_horizon) * rplus;
     const T rprom2 = rprom * rprom;
     const T invM = T(1) / M;
@@ -520,17 +533,19 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     T rk_min = T(1e30);
     // previous step start radius and velocity (escape-direction interpolation)
     T r_start = T(-1), v0x = T(0), v0y = T(0), v0z = T(0);
-    bool hit = false, nonfinite = false, escaped = false, routed = false;
+    bool hit = no_sky, nonfinite = false, escaped = false, routed = false;
     int steps = 0;
     const int order = P.order;
-    for (int it = 0; it < P.nmax; ++it) {
+    const int nmax = no_sky ? 0 : P.nmax;
+    for (int it = 0; it < nmax; ++it) {
         T rk2;
         const S6<T> k1 = rhs<T, KERR>(s, pt, M, a, rk2);
         const T r2 = s.x * s.x + s.y * s.y + s.z * s.z;
         const T r = (sizeof(T) == 4) ? r2 * A::rsq(r2) : A::sqrt(r2);
         if (P.rec) rk_min = mn(rk_min, rk2);
         if (P.capture_mode == 1) {
-            if (rk2 < rcap2) { hit = true; break; }
+            if ((rk2 < rcap2 && rk2 < rk2_prev) || rk2 < rfloor2) { hit = true; break; }
+            rk2_prev = rk2;
         } else if (KERR) {
             if (rk2 < rkill2) { hit = true; break; }
         } else if (r < T(3.2) * M) {
@@ -565,12 +580,12 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             accum = axpy(accum, T(2), k);
             k = rhs<T, KERR>(axpy(s, h, k), pt, M, a, rk2d);
             stage_min = mn(stage_min, rk2d);
-            if (P.capture_mode == 1 && stage_min < rcap2) { hit = true; break; }
+            if (P.capture_mode == 1 && stage_min < rcap2 && stage_min < rk2) { hit = true; break; }
             accum = axpy(accum, T(1), k);
             s = axpy(s, h * T(1.0 / 6.0), accum);
         } else if (order == 2) {
             const S6<T> k2 = rhs<T, KERR>(axpy(s, T(0.5) * h, k1), pt, M, a, rk2d);
-            if (P.capture_mode == 1 && rk2d < rcap2) { hit = true; break; }
+            if (P.capture_mode == 1 && rk2d < rcap2 && rk2d < rk2) { hit = true; break; }
             s = axpy(s, h, k2);
         } else {
             s = axpy(s, h, k1);

@@ -74,3 +74,64 @@ def sample_pixels(width: int, height: int, n: int = 4096, seed: int = SAMPLE_SEE
     rng = np.random.default_rng(seed)
     idx = rng.choice(width * height, size=n, replace=False)
     return np.stack([idx % width, idx // width], axis=1).astype(np.int32)
+
+
+def freefall_radii(n: int, r0: float = 30.0, r1: float = 1.5, M: float = 1.0) -> np.ndarray:
+    """Radii at n equal proper-time intervals of a radial fall from rest at r0
+    down to r1 (Schwarzschild cycloid r = r0/2 (1 + cos eta),
+    tau = sqrt(r0^3 / 8M) (eta + sin eta))."""
+    eta1 = np.arccos(2.0 * r1 / r0 - 1.0)
+    tau = lambda e: np.sqrt(r0 ** 3 / (8.0 * M)) * (e + np.sin(e))
+    taus = np.linspace(0.0, tau(eta1), n)
+    out = np.empty(n)
+    for k, t in enumerate(taus):
+        lo, hi = 0.0, eta1
+        for _ in range(80):
+            mid = 0.5 * (lo + hi)
+            if tau(mid) < t:
+                lo = mid
+            else:
+                hi = mid
+        out[k] = 0.5 * r0 * (1.0 + np.cos(0.5 * (lo + hi)))
+    return out
+
+
+BATCH_RESOLUTIONS = ((256, 144), (640, 360), (1280, 720), (1920, 1080))
+
+
+@dataclass
+class Batch:
+    spacetime: object
+    cameras: list
+    sizes: list
+    disc: api.AccretionDisc
+    rule: api.StepRule
+
+    def frame_descs(self):
+        descs, off = [], 0
+        for (w, h) in self.sizes:
+            descs.append((w, h, off))
+            off += 3 * w * h
+        return descs, off
+
+    def cam_blocks(self):
+        a = api.spin(self.spacetime)
+        c32 = np.stack([api.ks_cam_params(c, self.spacetime.M, a)[0] for c in self.cameras])
+        c64 = np.stack([api.ks_cam_params(c, self.spacetime.M, a)[1] for c in self.cameras])
+        return np.ascontiguousarray(c32), np.ascontiguousarray(c64)
+
+
+def batch_freefall(n: int = 64, seed: int = 64, resolutions=BATCH_RESOLUTIONS, scale: float = 1.0) -> Batch:
+    """n cameras on a radial free fall r = 30M -> 1.5M (through the horizon of
+    a Schwarzschild hole), each boosted by |v| ~ U(0, 0.6c) toward the hole,
+    frame sizes drawn uniformly from `resolutions` (scaled by `scale`)."""
+    rng = np.random.default_rng(seed)
+    d = np.array([0.0, -1.0, 0.1])
+    d /= np.linalg.norm(d)
+    cams, sizes = [], []
+    for r in freefall_radii(n):
+        v = rng.uniform(0.0, 0.6)
+        cams.append(api.Camera(r * d, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0), velocity=-v * d))
+        w, h = resolutions[rng.integers(0, len(resolutions))]
+        sizes.append((max(8, int(w * scale)), max(8, int(h * scale))))
+    return Batch(api.Schwarzschild(1.0), cams, sizes, config_disc(), api.StepRule.fixed_fraction(0.05, 100.0, 1.01))

@@ -1,0 +1,78 @@
+"""Learned surrogate of the lensing map (no counterpart in the reference).
+
+An MLP 5 -> 256 -> 256 -> 256 -> 256 -> 5 (SiLU), BF16 weights and FP32
+accumulation on the MI355X's MFMA units, maps a static/ZAMO camera's
+(r, theta, a) and a ray's local angles (alpha, beta) to the ray's escape
+direction and fate.  It is trained on rays the RK4 kernel traces; RK4 is
+also its fallback and the judge of its error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from . import api
+
+
+class Surrogate:
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        L.check(L.lib().st_mlp_create(C.byref(h), int(seed)))
+        self._h = h
+        self.n_params = int(L.lib().st_mlp_param_count())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().st_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        L.check(L.lib().st_mlp_get_params(self._h, L.fptr(out), out.size))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        L.check(L.lib().st_mlp_set_params(self._h, L.fptr(p), p.size))
+
+    def grad(self, x: np.ndarray, t: np.ndarray, w_dir: float = 1.0):
+        """Forward + backward on one batch: (outputs (n, 5), gradient, loss sums)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = x.shape[0]
+        y = np.empty((n, 5), dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        l3 = np.empty(3, dtype=np.float32)
+        L.check(L.lib().st_mlp_grad(self._h, L.fptr(x), L.fptr(t), n, L.fptr(y), L.fptr(g), float(w_dir), L.fptr(l3)))
+        return y, g, l3
+
+    def generate(self, ctx: api.HipPreviewContext, n_cameras: int, side: int = 32, r_range=(4.0, 50.0),
+                 a_range=(0.0, 0.99), fov: float = 1.4, seed: int = 7,
+                 rule: Optional[api.StepRule] = None):
+        """Trace n_cameras * side^2 training rays with the RK4 kernel."""
+        rule = rule if rule is not None else api.StepRule.fixed_fraction(0.05, 100.0, 1.01)
+        dom = L.SurrogateDomain(n_cameras, side, r_range[0], r_range[1], a_range[0], a_range[1], fov, seed)
+        cam = api.Camera((0.0, -30.0, 1.0), (0, 0, 0), (0, 0, 1), fov)
+        sp = api.spacetime_params(api.Kerr(1.0, 0.9), cam, rule)
+        args = api.trace_args(ctx.width, ctx.height, rule)
+        L.check(L.lib().st_mlp_generate(self._h, ctx.handle, C.byref(dom), L.fptr(sp), C.byref(args)))
+
+    def train(self, steps: int, batch: int = 8192, lr: float = 2e-3, lr_final: float = 2e-5, warmup: int = 200,
+              w_dir: float = 300.0, clip: float = 1.0, seed: int = 11) -> np.ndarray:
+        o = L.TrainOpts(steps, batch, warmup, lr, lr_final, w_dir, clip, seed)
+        loss = np.empty((steps, 3), dtype=np.float32)
+        L.check(L.lib().st_mlp_train(self._h, C.byref(o), L.fptr(loss)))
+        return loss

@@ -1,0 +1,45 @@
+"""The surrogate's BF16 MFMA training kernels against the FP64 numpy oracle
+(oracle/mlp.py): outputs and every parameter block's gradient on one batch.
+Tolerance: BF16 storage of weights/activations/back-propagated errors
+(8-bit significand) with FP32 accumulation -> relative L2 error <= 3e-2 per
+block, cosine >= 0.999."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp as OM
+from spacetime_amd import surrogate
+from test_mlp_oracle import random_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradient_match_fp64_oracle(hip):
+    rng = np.random.default_rng(5)
+    P, X, T = random_problem(rng, B=256)
+    w_dir = 50.0
+    print("case: mlp gradient check B=256", flush=True)
+    s = surrogate.Surrogate(seed=1)
+    s.set_params(P.astype(np.float32))
+    P32 = s.params().astype(np.float64)
+    y, g, l3 = s.grad(X, T, w_dir)
+    L, gref, yref = OM.loss_grad(P32, X.astype(np.float32).astype(np.float64), T.astype(np.float32), w_dir)
+    err_y = np.linalg.norm(y - yref) / np.linalg.norm(yref)
+    print(f"forward rel L2 {err_y:.3e}; loss gpu {(l3[0] + l3[1]) / 256:.6f} ref {L:.6f}")
+    assert err_y < 2e-2
+    assert abs((l3[0] + l3[1]) / 256 - L) < 3e-2 * abs(L)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4", "W5", "b5"):
+        a, n = OM.OFF[k]
+        if k == "W5":
+            n = OM.NOUT * OM.HID
+        if k == "b5":
+            n = OM.NOUT
+        gg, gr = g[a:a + n].astype(np.float64), gref[a:a + n]
+        rel = np.linalg.norm(gg - gr) / np.linalg.norm(gr)
+        cos = gg @ gr / (np.linalg.norm(gg) * np.linalg.norm(gr))
+        print(f"  {k}: rel L2 {rel:.3e} cos {cos:.6f}")
+        assert rel < 3e-2 and cos > 0.999, k
+    a5, _ = OM.OFF["W5"]
+    assert np.all(g[a5 + OM.NOUT * OM.HID:a5 + OM.NOUTP * OM.HID] == 0)
+    s.close()
