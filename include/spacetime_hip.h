@@ -97,6 +97,7 @@ typedef struct {
     float promote_band;    /* FP64 promotion: |r_peri - r_ph| < band (M units) */
     float promote_horizon; /* FP64 promotion: KS r < promote_horizon * r_plus */
     int stat;              /* reserved: refinement statistics variant */
+    float gas_ds;          /* gas sample spacing in arc length (0: the reference's stride * dt) */
 } st_trace_args;
 
 /* Counters of the last st_trace / st_trace_batch on a context. */
@@ -237,6 +238,26 @@ int st_mlp_generate(st_mlp* mlp, st_ctx* ctx, const st_surrogate_domain* dom, co
                     const st_trace_args* args);
 /* Adam steps on random mini-batches; loss_out[steps][3] (may be NULL). */
 int st_mlp_train(st_mlp* mlp, const st_train_opts* opts, float* loss_out);
+
+/* A frame for the surrogate: pinhole camera at pos looking at the hole with
+ * up +z (the surrogate's camera family), fov_factor as in Camera. */
+typedef struct {
+    double pos[3];
+    double fov;
+    float conf;            /* fall back to RK4 below this fate confidence (0.99) */
+    float delta_max;       /* ... or above this predicted deflection correction (rad) */
+    int edge_px;           /* ... or within this many pixels of the other predicted fate (2) */
+} st_surrogate_view;
+
+/* Surrogate + fallback frame into the context's buffer (out += weight *
+ * sample, records if enabled): fused MLP inference for every pixel, then
+ * shading of the confident pixels and FP32 RK4 for the rest (the
+ * context's scene and step rule; promoted = fallback count in st_stats). */
+int st_mlp_render(st_mlp* mlp, st_ctx* ctx, const st_surrogate_view* view, const float spacetime_params[8],
+                  const st_trace_args* args);
+/* The fused inference alone: pred[H][W][8] = (direction xyz, P(escape),
+ * |correction|, 0, 0, 0) for the view on the context's frame size. */
+int st_mlp_infer(st_mlp* mlp, st_ctx* ctx, const st_surrogate_view* view, float spin, float* pred_host);
 
 /* ---- host helpers (FP64 host math, no device work) ---- */
 /* _ks_cam_params (src/metal.jl:1760-1774) for Camera(pos, target, up, fov;

@@ -789,7 +789,7 @@ void ray_start(const ora_scene* sc, int i, int j, ora_ray* o, RayStart& rs) {
      * inside it starts with rays moving outward */
     rs.rk2_prev0 = ks_r2(sc, rs.s) > rs.rcap * rs.rcap ? 1e300 : 0.0;
     /* E = p_t <= 0: the ray cannot have come from infinity (capture mode 1) */
-    rs.no_sky = sc->capture_mode == 1 && !(rs.p_t > 0.0);
+    rs.no_sky = sc->capture_mode == 1 && rs.p_t <= 0.0;
     const double r0 = std::sqrt(rs.s[0] * rs.s[0] + rs.s[1] * rs.s[1] + rs.s[2] * rs.s[2]);
     rs.r_floor = r0 > 2.05 * sc->M ? 2.0 * sc->M : 0.3 * sc->M;
 }
@@ -821,7 +821,7 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
     const double vol_s_out = std::exp(sc->vol_params[1]);
     const double vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
     const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
-    const double ds_gas = mstep * sc->dt;
+    const double ds_gas = sc->gas_ds > 0.0 ? sc->gas_ds : mstep * sc->dt;
     int steps = 0;
     for (int it = 0; it < (rs.no_sky ? 0 : sc->nmax); ++it) {
         const double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
@@ -980,7 +980,7 @@ int ora_trace_tsit5(const ora_scene* sc, int i, int j, double rtol, double atol,
     const double vol_s_out = std::exp(sc->vol_params[1]);
     const double vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
     const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
-    const double ds_gas = mstep * sc->dt;
+    const double ds_gas = sc->gas_ds > 0.0 ? sc->gas_ds : mstep * sc->dt;
     bool hit = rs.no_sky, nonfinite = false;
     o->rk_min = std::sqrt(std::max(ks_r2(sc, s), 0.0));
     if (hit) {

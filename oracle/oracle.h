@@ -68,6 +68,7 @@ typedef struct {
     const float* vol;
     int nr, nphi, nz;
     double vol_params[9];
+    double gas_ds;         /* gas sample spacing (0: stride * dt) */
     /* camera */
     double cam[49];
     int width, height;

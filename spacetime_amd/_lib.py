@@ -37,7 +37,7 @@ class TraceArgs(C.Structure):
         ("h_floor", C.c_float), ("capture_mode", C.c_int), ("capture_factor", C.c_float),
         ("kerr_shrink", C.c_int), ("esc_interp", C.c_int), ("bardeen", C.c_int),
         ("precision", C.c_int), ("promote_band", C.c_float), ("promote_horizon", C.c_float),
-        ("stat", C.c_int),
+        ("stat", C.c_int), ("gas_ds", C.c_float),
     ]
 
 
@@ -58,6 +58,11 @@ class FrameDesc(C.Structure):
 class SurrogateDomain(C.Structure):
     _fields_ = [("n_cameras", C.c_int), ("side", C.c_int), ("r_min", C.c_float), ("r_max", C.c_float),
                 ("a_min", C.c_float), ("a_max", C.c_float), ("fov", C.c_float), ("seed", C.c_uint64)]
+
+
+class SurrogateView(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("fov", C.c_double), ("conf", C.c_float), ("delta_max", C.c_float),
+                ("edge_px", C.c_int)]
 
 
 class TrainOpts(C.Structure):
@@ -100,6 +105,8 @@ SIGNATURES = {
     "st_mlp_grad": (C.c_int, [C.c_void_p, _F, _F, C.c_int, _F, _F, C.c_float, _F]),
     "st_mlp_generate": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateDomain), _F, C.POINTER(TraceArgs)]),
     "st_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(TrainOpts), _F]),
+    "st_mlp_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateView), _F, C.POINTER(TraceArgs)]),
+    "st_mlp_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SurrogateView), C.c_float, _F]),
     "st_camera_params": (C.c_int, [_D, _D, _D, C.c_double, _D, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, _D]),
     "st_camera_params_basis": (C.c_int, [_D, _D, _D, _D, C.c_double, _D, C.c_double, C.c_double, _D]),

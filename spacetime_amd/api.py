@@ -246,7 +246,7 @@ def spacetime_params(st, cam: Camera, rule: StepRule, relativistic: bool = False
 
 def trace_args(width: int, height: int, rule: StepRule, *, precision: int = L.ST_PREC_FP32, ju=0.5, jv=0.5,
                jw=0.0, jseed=0, weight=1.0, row0=0, rows=None, col0=0, cols=-1,
-               promote_band=0.05, promote_horizon=1.1) -> L.TraceArgs:
+               promote_band=0.05, promote_horizon=1.1, gas_ds=0.0) -> L.TraceArgs:
     a = L.default_args(width, height)
     a.nmax = int(rule.nmax)
     a.dt = rule.dt
@@ -265,6 +265,7 @@ def trace_args(width: int, height: int, rule: StepRule, *, precision: int = L.ST
     a.precision = precision
     a.promote_band = promote_band
     a.promote_horizon = promote_horizon
+    a.gas_ds = gas_ds
     return a
 
 

@@ -92,6 +92,7 @@ int st::fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceP
     P.capture_mode = a->capture_mode; P.capture_factor = a->capture_factor;
     P.kerr_shrink = a->kerr_shrink; P.esc_interp = a->esc_interp; P.bardeen = a->bardeen;
     P.promote_band = a->promote_band; P.promote_horizon = a->promote_horizon;
+    P.gas_ds = a->gas_ds;
     P.assign = 0;
     P.queue = c->queue; P.qcount = c->qcount; P.counters = c->counters;
     return ST_OK;
@@ -139,6 +140,7 @@ void st_trace_args_default(st_trace_args* a, int width, int height) {
     a->promote_band = 0.05f;
     a->promote_horizon = 1.1f;
     a->stat = 0;
+    a->gas_ds = 0.0f;
 }
 
 int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, int height) {
@@ -356,7 +358,7 @@ int st_get_stats(st_ctx* c, st_stats* s) {
     ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
     ST_HIP(hipMemcpy(&q, c->qcount, sizeof(q), hipMemcpyDeviceToHost));
     s->rays = cnt[3];
-    s->steps_fp32 = cnt[0];
+    s->steps_fp32 = cnt[0] + cnt[2];
     s->steps_fp64 = cnt[1];
     s->promoted = c->last_mixed ? q : 0;
     float ms = 0.0f;

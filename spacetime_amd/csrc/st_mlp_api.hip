@@ -38,6 +38,11 @@ struct st_mlp {
     float* loss = nullptr;      // [steps][3]
     int loss_cap = 0;
     float* sumsq = nullptr;
+    // inference
+    bf16 *W2f = nullptr, *W3f = nullptr, *W4f = nullptr, *W5f = nullptr;
+    bool frag_dirty = true;
+    float* pred = nullptr;
+    size_t pred_cap = 0;
 };
 
 namespace {
@@ -48,6 +53,55 @@ void dfree(void* p) {
 
 int refresh(st_mlp* m) {
     ST_HIP(launch_refresh(m->P, m->W2b, m->W2t, m->W3b, m->W3t, m->W4b, m->W4t, m->W5b, m->W5t, m->stream));
+    m->frag_dirty = true;
+    return ST_OK;
+}
+
+// Fragment-ordered inference weights and the per-pixel prediction buffer.
+int prepare_infer(st_mlp* m, st_ctx* c, size_t npix) {
+    if (!m->W2f) {
+        ST_HIP(hipMalloc(&m->W2f, (size_t)HID * HID * sizeof(bf16)));
+        ST_HIP(hipMalloc(&m->W3f, (size_t)HID * HID * sizeof(bf16)));
+        ST_HIP(hipMalloc(&m->W4f, (size_t)HID * HID * sizeof(bf16)));
+        ST_HIP(hipMalloc(&m->W5f, (size_t)16 * HID * sizeof(bf16)));
+    }
+    if (m->frag_dirty) {
+        ST_HIP(hipStreamSynchronize(m->stream));
+        ST_HIP(launch_fragment_weights(m->P, m->W2f, m->W3f, m->W4f, m->W5f, c->stream));
+        m->frag_dirty = false;
+    }
+    if (m->pred_cap < npix) {
+        dfree(m->pred);
+        m->pred = nullptr;
+        ST_HIP(hipMalloc(&m->pred, npix * 8 * sizeof(float)));
+        m->pred_cap = npix;
+    }
+    return ST_OK;
+}
+
+int make_view(const st_surrogate_view* v, float a, int width, int height, float ju, float jv, InferView& V,
+              double cam[49]) {
+    const double* p = v->pos;
+    const double rho2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    const double w = rho2 - (double)a * a;
+    const double r = std::sqrt(0.5 * (w + std::sqrt(w * w + 4.0 * (double)a * a * p[2] * p[2])));
+    if (!(r > 0.0)) return fail(ST_ERR_INVALID_ARG, "camera at the origin");
+    double f[3] = {-p[0], -p[1], -p[2]};
+    const double fn = std::sqrt(rho2);
+    for (double& x : f) x /= fn;
+    double rr[3] = {f[1], -f[0], 0.0};   // f x z
+    const double rn = std::sqrt(rr[0] * rr[0] + rr[1] * rr[1]);
+    if (!(rn > 1e-9)) return fail(ST_ERR_INVALID_ARG, "camera on the spin axis (up +z is degenerate)");
+    for (double& x : rr) x /= rn;
+    const double uu[3] = {rr[1] * f[2] - rr[2] * f[1], rr[2] * f[0] - rr[0] * f[2], rr[0] * f[1] - rr[1] * f[0]};
+    V.width = width; V.height = height;
+    V.fov = (float)v->fov; V.ju = ju; V.jv = jv;
+    V.r = (float)r; V.th = (float)std::acos(std::min(std::max(p[2] / r, -1.0), 1.0)); V.a = a;
+    for (int q = 0; q < 3; ++q) { V.axes[q] = (float)rr[q]; V.axes[3 + q] = (float)uu[q]; V.axes[6 + q] = (float)f[q]; }
+    V.conf = v->conf; V.delta_max = v->delta_max; V.edge_px = v->edge_px;
+    const double tgt[3] = {0.0, 0.0, 0.0}, up[3] = {0.0, 0.0, 1.0}, vel[3] = {0.0, 0.0, 0.0};
+    const int rc = st_camera_params(p, tgt, up, v->fov, vel, 1.0, a, 0.0, 1.0, 0, cam);
+    if (rc != ST_OK) return fail(rc, "camera parameters");
     return ST_OK;
 }
 
@@ -185,6 +239,7 @@ int st_mlp_destroy(st_mlp* m) {
     dfree(m->Xb); dfree(m->Tb); dfree(m->Y); dfree(m->dH); dfree(m->dZ1f);
     for (int l = 1; l <= 4; ++l) { dfree(m->Z[l]); dfree(m->H[l]); dfree(m->Ht[l]); }
     dfree(m->dY); dfree(m->dYt); dfree(m->dZ); dfree(m->dZt);
+    dfree(m->W2f); dfree(m->W3f); dfree(m->W4f); dfree(m->W5f); dfree(m->pred);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return ST_OK;
@@ -349,6 +404,55 @@ int st_mlp_train(st_mlp* m, const st_train_opts* o, float* loss_out) {
     ST_HIP(hipStreamSynchronize(m->stream));
     if (loss_out)
         ST_HIP(hipMemcpy(loss_out, m->loss, sizeof(float) * 3 * (size_t)o->steps, hipMemcpyDeviceToHost));
+    return ST_OK;
+}
+
+int st_mlp_infer(st_mlp* m, st_ctx* c, const st_surrogate_view* v, float spin, float* pred_host) {
+    if (!m || !c || !v || !pred_host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    const int npix = c->width * c->height;
+    InferView V;
+    double cam[49];
+    int rc = make_view(v, spin, c->width, c->height, 0.5f, 0.5f, V, cam);
+    if (rc != ST_OK) return rc;
+    if ((rc = prepare_infer(m, c, (size_t)npix)) != ST_OK) return rc;
+    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, npix, c->stream));
+    ST_HIP(hipMemcpyAsync(pred_host, m->pred, (size_t)npix * 8 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_mlp_render(st_mlp* m, st_ctx* c, const st_surrogate_view* v, const float sp[8], const st_trace_args* a) {
+    if (!m || !c || !v || !sp || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (a->width != c->width || a->height != c->height)
+        return fail(ST_ERR_INVALID_ARG, "args width/height must match the context");
+    if (a->precision != ST_PREC_FP32) return fail(ST_ERR_UNSUPPORTED, "surrogate fallback runs the FP32 path");
+    if (c->has_volume && c->vol_on) return fail(ST_ERR_UNSUPPORTED, "the surrogate has no gas model");
+    const int npix = c->width * c->height;
+    st::TraceParams P;
+    int rc = st::fill_params(c, sp, a, P);
+    if (rc != ST_OK) return rc;
+    InferView V;
+    double cam[49];
+    if ((rc = make_view(v, sp[6], c->width, c->height, a->ju, a->jv, V, cam)) != ST_OK) return rc;
+    if ((rc = prepare_infer(m, c, (size_t)npix)) != ST_OK) return rc;
+    st::DeviceScene D = st::make_scene(c);
+    st::FrameRef F;
+    std::memset(&F, 0, sizeof(F));
+    for (int q = 0; q < 49; ++q) { F.c32[q] = (float)cam[q]; F.c64[q] = cam[q]; }
+    F.has64 = 1;
+    F.width = c->width; F.height = c->height;
+    F.out = c->out;
+    F.rec = c->rec_on ? c->rec : nullptr;
+    const bool kerr = sp[6] != 0.0f;
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
+    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, npix, c->stream));
+    ST_HIP(launch_select(m->pred, V, D, c->out, F.rec, a->weight, c->queue, c->qcount, npix, c->stream));
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(st::launch_queue32(kerr, false, D, P, F, c->num_cu * 4, c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = true;
+    c->have_last = true;
     return ST_OK;
 }
 

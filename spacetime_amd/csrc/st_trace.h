@@ -40,6 +40,7 @@ struct TraceParams {
     float capture_factor;
     int kerr_shrink, esc_interp, bardeen;
     float promote_band, promote_horizon;
+    float gas_ds;            // gas sample spacing (0: stride * dt)
     int assign;              // 1: out = sample (batched frames); 0: out += weight * sample
     float* out;
     float* rec;              // ST_REC_N floats per pixel, or null
@@ -382,6 +383,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     const int I = i + 1, J = j + 1;   // the reference hashes 1-based pixel indices
     const T M = T(P.M), a = KERR ? T(F.has_spin ? F.spin : P.a) : T(0);
     o.routed = false;
+    o.steps = 0;
     o.flags = (sizeof(T) == 8) ? 1 : 0;
 
     // ---- pose (optional per-pixel shutter, src/metal.jl:833-848) ----
@@ -481,7 +483,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     // t component) is not positive cannot have come from infinity: from a
     // camera inside the horizon it leads to the other exterior.  Capture
     // mode 1 paints it black without integrating.
-    const bool no_sky = P.capture_mode == 1 && !(pt > T(0));
+    const bool no_sky = P.capture_mode == 1 && pt <= T(0);   // false for NaN: that ray stays non-finite
     // ---- launch-time radial potential (src/metal.jl:1008-1094) ----
     bool captured0 = false;
     const T rplus = M + A::psqrt(mx(M * M - a * a, T(0)));
@@ -511,7 +513,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     const T vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
     int mstep = (int)D.vol_params[8];
     mstep = mstep < 1 ? 1 : (mstep > 16 ? 16 : mstep);
-    const T ds_gas = T(mstep) * T(P.dt);
+    const T ds_gas = P.gas_ds > 0.0f ? T(P.gas_ds) : T(mstep) * T(P.dt);
     const T r_escape = T(P.r_escape), dt = T(P.dt), h_coef = T(P.h_coef), h_floor = T(P.h_floor);
     const T h_cap = T(P.h_cap), rkill2 = T(P.rkill2);
     // Capture mode 1: a ray below the capture radius and moving inward is

@@ -76,3 +76,25 @@ class Surrogate:
         loss = np.empty((steps, 3), dtype=np.float32)
         L.check(L.lib().st_mlp_train(self._h, C.byref(o), L.fptr(loss)))
         return loss
+
+    @staticmethod
+    def view(pos, fov, conf=0.99, delta_max=1.0, edge_px=2) -> L.SurrogateView:
+        v = L.SurrogateView()
+        for k in range(3):
+            v.pos[k] = float(pos[k])
+        v.fov, v.conf, v.delta_max, v.edge_px = float(fov), float(conf), float(delta_max), int(edge_px)
+        return v
+
+    def infer(self, ctx: api.HipPreviewContext, pos, fov, spin: float) -> np.ndarray:
+        """Fused inference alone: (H, W, 8) = (direction, P(escape), |correction|, 0...)."""
+        out = np.empty((ctx.height, ctx.width, 8), dtype=np.float32)
+        v = self.view(pos, fov)
+        L.check(L.lib().st_mlp_infer(self._h, ctx.handle, C.byref(v), float(spin), L.fptr(out)))
+        return out
+
+    def render(self, ctx: api.HipPreviewContext, pos, fov, sp: np.ndarray, args: L.TraceArgs, conf=0.99,
+               delta_max=1.0, edge_px=2):
+        """Surrogate + RK4-fallback frame accumulated into the context's buffer."""
+        v = self.view(pos, fov, conf, delta_max, edge_px)
+        L.check(L.lib().st_mlp_render(self._h, ctx.handle, C.byref(v),
+                                      L.fptr(np.ascontiguousarray(sp, dtype=np.float32)), C.byref(args)))

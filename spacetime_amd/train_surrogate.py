@@ -1,0 +1,76 @@
+"""Train the lensing-map surrogate on rays the RK4 kernel traces and save its
+weights.
+
+    python -m spacetime_amd.train_surrogate --rays 50e6 --steps 6000 \
+        --out spacetime_amd/surrogate_weights.f32
+
+The training set covers cameras at r in [4, 50] M, theta in (0, pi), spin in
+[0, 0.99], each tracing side x side jittered rays with tan(alpha),
+tan(beta) in [-1.4, 1.4].  Progress is printed every `--report` steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import api, scenes, surrogate
+
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "surrogate_weights.f32"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rays", type=float, default=50e6)
+    ap.add_argument("--side", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--w-dir", type=float, default=300.0)
+    ap.add_argument("--report", type=int, default=500)
+    ap.add_argument("--seed", type=int, default=20250101)
+    ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    ap.add_argument("--init", default=None, help="start from these weights")
+    args = ap.parse_args(argv)
+
+    s = surrogate.Surrogate(seed=args.seed)
+    if args.init:
+        s.set_params(np.fromfile(args.init, dtype=np.float32))
+    ctx = api.HipPreviewContext(scenes.sky(), 8, 8)
+    ncam = int(round(args.rays / (args.side * args.side)))
+    t0 = time.time()
+    s.generate(ctx, n_cameras=ncam, side=args.side, seed=args.seed + 1)
+    print(f"generated {ncam * args.side ** 2:.3e} rays from {ncam} cameras in {time.time() - t0:.1f}s", flush=True)
+    hist = []
+    done = 0
+    t0 = time.time()
+    while done < args.steps:
+        n = min(args.report, args.steps - done)
+        # one cosine schedule over the whole run, continued chunk by chunk
+        frac0, frac1 = done / args.steps, (done + n) / args.steps
+        lr0 = 2e-5 + 0.5 * (args.lr - 2e-5) * (1 + np.cos(np.pi * frac0))
+        lr1 = 2e-5 + 0.5 * (args.lr - 2e-5) * (1 + np.cos(np.pi * frac1))
+        loss = s.train(n, batch=args.batch, lr=float(lr0), lr_final=float(lr1), warmup=200 if done == 0 else 0,
+                       w_dir=args.w_dir, seed=args.seed + 7 * done)
+        done += n
+        per = (loss[:, 0] + loss[:, 1]) / np.maximum(loss[:, 2], 1)
+        dirl = loss[:, 0] / args.w_dir / np.maximum(loss[:, 2], 1)
+        hist.append((done, float(per[-n // 4:].mean()), float(dirl[-n // 4:].mean())))
+        print(f"step {done}: loss/sample {per[-n // 4:].mean():.5f}  dir mse {dirl[-n // 4:].mean():.3e}  "
+              f"({time.time() - t0:.0f}s)", flush=True)
+    p = s.params()
+    p.tofile(args.out)
+    Path(args.out).with_suffix(".json").write_text(json.dumps({
+        "rays": ncam * args.side ** 2, "cameras": ncam, "steps": args.steps, "batch": args.batch,
+        "train_seconds": time.time() - t0, "history": hist}, indent=1))
+    print(f"saved {args.out}", flush=True)
+    ctx.close()
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""Surrogate lensing map on the GPU: the fused BF16 MFMA inference kernel
+against the training path's forward pass, the fallback path against plain
+RK4, and a short training run on rays the RK4 kernel generates."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from spacetime_amd import _lib as L
+from spacetime_amd import api, scenes, surrogate
+
+pytestmark = pytest.mark.gpu
+
+POS = (0.0, -30.0, 3.0)
+FOV = 0.75
+
+
+def features(r, th, a, al, be):
+    """The network's input scaling (the library's mlp::features)."""
+    return np.stack([(r - 27.0) / 23.0, (th - math.pi / 2) / (math.pi / 2), np.full_like(al, (a - 0.495) / 0.495),
+                     al / 0.95, be / 0.95], axis=-1)
+
+
+def view_inputs(W, H, a):
+    p = np.array(POS)
+    w = p @ p - a * a
+    r = math.sqrt(0.5 * (w + math.sqrt(w * w + 4 * a * a * p[2] ** 2)))
+    th = math.acos(p[2] / r)
+    i, j = np.meshgrid(np.arange(W), np.arange(H))
+    u = (i + 0.5 - W / 2) / (H / 2)
+    v = (j + 0.5 - H / 2) / (H / 2)
+    tu, tv = u * FOV, v * FOV
+    x = features(np.full(tu.shape, r), np.full(tu.shape, th), a, np.arctan(tu), np.arctan(tv))
+    inu = 1 / np.sqrt(tu * tu + tv * tv + 1)
+    n = np.stack([tu * inu, tv * inu, inu], axis=-1)
+    f = -p / np.linalg.norm(p)
+    rr = np.cross(f, [0, 0, 1.0]); rr /= np.linalg.norm(rr)
+    uu = np.cross(rr, f)
+    return x.reshape(-1, 5), n.reshape(-1, 3), np.stack([rr, uu, f])
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A small surrogate trained briefly (enough to make outputs non-trivial)."""
+    s = surrogate.Surrogate(seed=3)
+    ctx = api.HipPreviewContext(scenes.sky(), 8, 8)
+    print("case: surrogate generate 256 cameras x 32^2 + 200 steps", flush=True)
+    s.generate(ctx, n_cameras=256, side=32, seed=5)
+    loss = s.train(200, batch=4096, lr=2e-3, warmup=20)
+    ctx.close()
+    return s, loss
+
+
+def test_training_reduces_loss(hip, trained):
+    s, loss = trained
+    per = (loss[:, 0] + loss[:, 1]) / np.maximum(loss[:, 2], 1)
+    first, last = per[:10].mean(), per[-10:].mean()
+    print(f"loss per sample: first {first:.4f} last {last:.4f}")
+    assert np.all(np.isfinite(per)) and last < 0.5 * first
+
+
+def test_fused_inference_matches_training_forward(hip, trained):
+    s, _ = trained
+    W, H, a = 64, 36, 0.9
+    ctx = api.HipPreviewContext(scenes.sky(), W, H)
+    print("case: fused inference vs training forward", flush=True)
+    pred = s.infer(ctx, POS, FOV, a).reshape(-1, 8)
+    ctx.close()
+    x, n, axes = view_inputs(W, H, a)
+    t = np.zeros((x.shape[0], 8), np.float32)
+    y, _, _ = s.grad(x.astype(np.float32), t)
+    c = n + y[:, :3]
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    d = c @ axes
+    pe = 1 / (1 + np.exp(y[:, 4] - y[:, 3]))
+    ang = np.arccos(np.clip(np.sum(d * pred[:, :3], axis=1), -1, 1))
+    print(f"direction diff max {ang.max():.2e}, P(escape) diff max {np.abs(pe - pred[:, 3]).max():.2e}")
+    assert ang.max() < 2e-3
+    assert np.abs(pe - pred[:, 3]).max() < 1e-2
+
+
+def _plain_and_surrogate(s, conf, W=160, H=90, a=0.9):
+    sc = scenes.kerr09_1080p(W, H)
+    sc.disc = api.AccretionDisc(inner_radius=0.0, outer_radius=0.0)
+    ctx = api.HipPreviewContext(scenes.sky(), W, H)
+    ctx.enable_records(True)
+    c32, c64 = sc.cam_params()
+    sp = sc.spacetime_params()
+    args = api.trace_args(W, H, sc.rule)
+    ctx.clear()
+    ctx.trace(c32, c64, sp, args)
+    plain, prec = ctx.download(), ctx.records()
+    ctx.clear()
+    s.render(ctx, POS, FOV, sp, args, conf=conf)
+    sur, srec = ctx.download(), ctx.records()
+    st = ctx.stats()
+    ctx.close()
+    return plain, prec, sur, srec, st
+
+
+def test_all_fallback_equals_plain_rk4(hip, trained):
+    s, _ = trained
+    print("case: surrogate render with every pixel on the fallback path", flush=True)
+    plain, prec, sur, srec, st = _plain_and_surrogate(s, conf=1.01)
+    assert st["promoted"] == plain.shape[0] * plain.shape[1]
+    assert np.array_equal(srec[:, :, L.REC_FATE], prec[:, :, L.REC_FATE])
+    # the queue kernel and the tile kernel are separate compilations of the
+    # same FP32 tracer: equal up to FMA-contraction rounding
+    rel = np.abs(sur - plain).max(axis=2) / np.maximum(np.abs(plain).max(axis=2), 1e-6)
+    assert np.mean(rel <= 1e-6) >= 0.999 and rel.max() <= 1e-3
+
+
+def test_partial_fallback_consistency(hip, trained):
+    s, _ = trained
+    print("case: surrogate render with confidence 0.9", flush=True)
+    plain, prec, sur, srec, st = _plain_and_surrogate(s, conf=0.9)
+    flags = srec[:, :, L.REC_FLAGS].astype(int)
+    surr = (flags & 8) != 0
+    n = plain.shape[0] * plain.shape[1]
+    print(f"fallback {st['promoted']} of {n}")
+    assert st["promoted"] + surr.sum() == n
+    # fallback pixels are the RK4 pixels
+    rel = np.abs(sur[~surr] - plain[~surr]).max(axis=1) / np.maximum(np.abs(plain[~surr]).max(axis=1), 1e-6)
+    assert np.mean(rel <= 1e-6) >= 0.999 and rel.max() <= 1e-3
