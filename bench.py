@@ -3,7 +3,14 @@
 A step is one full-frame trace (2,073,600 rays) through libspacetime_hip's
 C ABI, camera/sky/LUT already resident in HBM.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-legs]
+    python bench.py --workload batch64     # 64 free-fall cameras, one launch
+    python bench.py --workload volume4k    # 4K Kerr a=0.99 gas disc, 8 frames/launch
+
+The default line also carries, measured in the same run: the mixed
+FP32/FP64 policy, the surrogate + RK4-fallback form (when trained weights
+are present) with its error against plain RK4, and the rate including the
+device-to-host copy of each frame.
 """
 from __future__ import annotations
 
@@ -29,23 +36,58 @@ F_RAY_KERR = 330                     # ray gen + KS init + Bardeen + ISCO + esca
 F_RAY_SCHW = 180
 FP32_PEAK_TFLOPS = 157.3             # MI355X FP32 vector, datasheet
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0            # MI355X dense BF16 MFMA, datasheet
+SURROGATE_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 5)   # 398,336
+METRIC = "Mrays/s at 1080p Kerr a=0.9 (plain RK4 vs surrogate+fallback) at ≤1e-3 rad error"
+WEIGHTS = ROOT / "spacetime_amd" / "surrogate_weights.f32"
+# surrogate gating used for the bench leg: fate confidence, largest predicted
+# correction to the pinhole direction kept (rad), shadow/ring edge margin (px)
+SUR_CONF, SUR_DELTA_MAX, SUR_EDGE = 0.99, 0.2, 2
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the tracer has no multi-GPU path in this build "
-              "(rays shard by frame/tile with no exchange; see the design notes).", file=sys.stderr)
-        sys.exit(2)
+def roofline(steps, rays, kernel_ms, f_step, f_ray, fp64=False):
+    flops = steps * f_step + rays * f_ray
+    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
+    tf = flops / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu-fp64" if fp64 else "valu-fp32", "achieved": round(tf, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(tf / peak, 4), "traffic": None,
+            "kernel_ms": round(kernel_ms, 4), "flop_per_launch": int(flops),
+            "steps_per_ray": round(steps / max(rays, 1), 3), "f_step": f_step, "f_ray": f_ray}
 
+
+def timed(fn, sync, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def header(args, W, H, rays_per_step, wall, workload, dtype):
+    return {
+        "metric": METRIC,
+        "value": round(rays_per_step / wall / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded 1024x512 star sky, analytic disc)",
+        "config": {"workload": workload, "rays_per_step": int(rays_per_step), "parallelism": "single GPU"},
+    }
+
+
+# ---------------------------------------------------------------------------
+# Kerr a=0.9 1080p frame (the headline)
+# ---------------------------------------------------------------------------
+def bench_frame(args):
     from spacetime_amd import _lib as L
     from spacetime_amd import api, scenes
 
@@ -60,68 +102,198 @@ def main():
     def step():
         ctx.trace(c32, c64, sp, targs)
 
-    for _ in range(args.warmup):
+    wall = timed(step, ctx.sync, args.steps, args.warmup)
+    # per-launch device time from HIP events on the context's stream, and
+    # the kernel's own step counters
+    kern, st = [], None
+    for _ in range(10):
         step()
-    ctx.sync()
-    # per-launch device time (HIP events on the context's stream) and steps
-    kern_ms = []
+        st = ctx.stats()
+        kern.append(st["ms_total"])
+    kernel_ms = float(np.mean(kern))
+    line = header(args, W, H, W * H, wall,
+                  f"kerr a=0.9 {W}x{H} fixed-fraction RK4 h=0.05r, r_esc=100M, capture 1.01 r+, thin disc + sky, "
+                  f"{args.precision}", {"fp32": "f32", "mixed": "f32+f64", "fp64": "f64"}[args.precision])
+    line["roofline"] = roofline(st["steps_fp32"] + st["steps_fp64"], st["rays"], kernel_ms, F_STEP_KERR, F_RAY_KERR,
+                                fp64=prec == L.ST_PREC_FP64)
+    line["roofline"]["kernel_ms_min"] = round(float(np.min(kern)), 4)
+    tfile = ROOT / "profiles" / "traffic_kerr09_1080p_fp32.json"
+    if tfile.exists() and prec == L.ST_PREC_FP32 and (W, H) == (1920, 1080):
+        line["roofline"]["traffic"] = json.loads(tfile.read_text()).get("bytes_per_launch")
+    line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
+
+    if not args.no_legs:
+        legs = {}
+        # the rate with each frame copied back to host memory (PCIe included)
+        out = np.empty((H, W, 3), np.float32)
+
+        def step_d2h():
+            ctx.trace(c32, c64, sp, targs)
+            ctx.download_into(out)
+
+        w2 = timed(step_d2h, ctx.sync, max(3, args.steps // 2), 1)
+        legs["incl_d2h"] = {"value": round(W * H / w2 / 1e6, 3), "unit": "Mrays/s",
+                            "note": "trace + 24.9 MB frame copied to host each step"}
+        if prec == L.ST_PREC_FP32:
+            margs = api.trace_args(W, H, sc.rule, precision=L.ST_PREC_MIXED)
+            wm = timed(lambda: ctx.trace(c32, c64, sp, margs), ctx.sync, max(3, args.steps // 2), 1)
+            ctx.trace(c32, c64, sp, margs)
+            sm = ctx.stats()
+            legs["mixed"] = {"value": round(W * H / wm / 1e6, 3), "unit": "Mrays/s",
+                             "promoted_fraction": round(sm["promoted"] / max(sm["rays"], 1), 5),
+                             "ms_fp32_pass": round(sm["ms_fp32"], 4), "ms_fp64_pass": round(sm["ms_fp64"], 4)}
+        if WEIGHTS.exists():
+            legs["surrogate_fallback"] = surrogate_leg(args, ctx, sc)
+        line["legs"] = legs
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(sc)
+    else:
+        line["cpu_baseline"] = None
+    return line
+
+
+def surrogate_leg(args, ctx, sc):
+    """Surrogate + RK4 fallback against plain FP32 RK4 on the same frame
+    without the disc (the surrogate maps sky directions only): both rates,
+    and the surrogate frame's direction error and fate agreement measured
+    against the plain RK4 frame over every pixel."""
+    from spacetime_amd import api, surrogate
+
+    W, H = sc.width, sc.height
+    sc2 = type(sc)(sc.name, sc.spacetime, sc.camera, api.AccretionDisc(inner_radius=0.0, outer_radius=0.0), sc.rule,
+                   W, H)
+    c32, c64 = sc2.cam_params()
+    sp = sc2.spacetime_params()
+    targs = api.trace_args(W, H, sc2.rule)
+    ctx.set_disc(sc2.disc)
+    s = surrogate.Surrogate(seed=1)
+    s.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
+    pos, fov = tuple(float(v) for v in sc2.camera.pos), float(sc2.camera.fov_factor)
+
+    def plain():
+        ctx.trace(c32, c64, sp, targs)
+
+    def sur():
+        s.render(ctx, pos, fov, sp, targs, conf=SUR_CONF, delta_max=SUR_DELTA_MAX, edge_px=SUR_EDGE)
+
+    n = max(3, args.steps // 2)
+    wp = timed(plain, ctx.sync, n, 1)
+    ws = timed(sur, ctx.sync, n, 1)
+    sur()
+    st = ctx.stats()
+    ctx.enable_records(True)
+    ctx.clear(); plain(); prec_ = ctx.records().reshape(-1, 12).copy()
+    ctx.clear(); sur(); srec = ctx.records().reshape(-1, 12).copy()
+    ctx.enable_records(False)
+    s.close()
+    both = (prec_[:, 0] == 0) & (srec[:, 0] == 0)
+    ang = np.arccos(np.clip(np.sum(prec_[both, 2:5] * srec[both, 2:5], axis=1), -1.0, 1.0))
+    kept = (srec[:, 10].astype(int) & 8) != 0
+    infer_tf = SURROGATE_FLOP_PER_RAY * W * H / (st["ms_fp32"] * 1e-3) / 1e12
+    return {
+        "value": round(W * H / ws / 1e6, 3), "unit": "Mrays/s",
+        "plain_rk4_no_disc": round(W * H / wp / 1e6, 3),
+        "speedup_vs_plain": round(wp / ws, 3),
+        "gating": {"conf": SUR_CONF, "delta_max_rad": SUR_DELTA_MAX, "edge_px": SUR_EDGE},
+        "fallback_fraction": round(float(1.0 - kept.mean()), 5),
+        "fate_agreement": round(float(np.mean(prec_[:, 0] == srec[:, 0])), 6),
+        "dir_err_rad": {"p50": float(np.percentile(ang, 50)), "p99": float(np.percentile(ang, 99)),
+                        "p99.9": float(np.percentile(ang, 99.9)), "max": float(ang.max())},
+        "meets_1e-3_rad_p99.9": bool(np.percentile(ang, 99.9) <= 1e-3),
+        "ms_inference_select": round(st["ms_fp32"], 4), "ms_fallback": round(st["ms_fp64"], 4),
+        "inference_roofline": {"bound": "mfma", "achieved": round(infer_tf, 1), "peak": BF16_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(infer_tf / BF16_PEAK_TFLOPS, 4),
+                               "note": "inference + select kernel time; 398,336 FLOP/ray"},
+    }
+
+
+# ---------------------------------------------------------------------------
+# 64 free-fall cameras of uneven resolution in one launch
+# ---------------------------------------------------------------------------
+def bench_batch(args):
+    from spacetime_amd import _lib as L
+    from spacetime_amd import api, scenes
+
+    prec = {"fp32": L.ST_PREC_FP32, "mixed": L.ST_PREC_MIXED}[args.precision]
+    b = scenes.batch_freefall()
+    ctx = api.HipPreviewContext(scenes.sky(), 8, 8, disc=b.disc)
+    c32, c64 = b.cam_blocks()
+    sp = api.spacetime_params(b.spacetime, b.cameras[0], b.rule)
+    bt = api.BatchTrace(ctx, c32, c64, b.sizes, sp)
+    targs = api.trace_args(8, 8, b.rule, precision=prec)
+    t, w, tune_ms = bt.autotune(targs)
+    rays = sum(x * y for x, y in b.sizes)
+
+    def step():
+        bt.trace(targs, graph=True)
+
+    wall = timed(step, ctx.sync, args.steps, args.warmup)
+    kern, st = [], None
     for _ in range(3):
         step()
         st = ctx.stats()
-        kern_ms.append(st["ms_total"])
-    steps_fp32, steps_fp64, rays, promoted = st["steps_fp32"], st["steps_fp64"], st["rays"], st["promoted"]
+        kern.append(st["ms_total"])
+    line = header(args, 0, 0, rays, wall,
+                  f"64 cameras on a radial free fall 30M->1.5M (Schwarzschild), v~U(0,0.6c), resolutions from "
+                  f"{{256x144,640x360,1280x720,1920x1080}}, one hipGraph launch, {args.precision}",
+                  {"fp32": "f32", "mixed": "f32+f64"}[args.precision])
+    line["metric"] = "Mrays/s, 64-camera batched free-fall trace"
+    line["config"]["autotuned"] = {"tile": t, "waves_per_cu": w, "ms": round(tune_ms, 3),
+                                   "one_shot": w == L.ST_BATCH_ONE_SHOT}
+    line["roofline"] = roofline(st["steps_fp32"] + st["steps_fp64"], st["rays"], float(np.mean(kern)), F_STEP_SCHW,
+                                F_RAY_SCHW)
+    line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
+    line["cpu_baseline"] = None
+    ctx.close()
+    return line
 
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+# ---------------------------------------------------------------------------
+# 3840x2160 Kerr a=0.99 with the volumetric disc, 8 orbit frames per launch
+# ---------------------------------------------------------------------------
+def bench_volume(args):
+    from spacetime_amd import _lib as L
+    from spacetime_amd import api, scenes
+
+    prec = {"fp32": L.ST_PREC_FP32, "mixed": L.ST_PREC_MIXED}[args.precision]
+    vol = scenes.disc_volume()
+    sc = scenes.kerr099_volume_4k()
+    cams = scenes.orbit_cameras()[:8]
+    ctx = api.HipPreviewContext(scenes.sky(), 8, 8, disc=sc.disc, volume=vol)
+    a = api.spin(sc.spacetime)
+    c32 = np.stack([api.ks_cam_params(c, sc.spacetime.M, a)[0] for c in cams])
+    c64 = np.stack([api.ks_cam_params(c, sc.spacetime.M, a)[1] for c in cams])
+    sp = api.spacetime_params(sc.spacetime, cams[0], sc.rule)
+    sizes = [(sc.width, sc.height)] * len(cams)
+    bt = api.BatchTrace(ctx, c32, c64, sizes, sp)
+    targs = api.trace_args(8, 8, sc.rule, precision=prec, gas_ds=scenes.GAS_DS_256)
+    t, w, tune_ms = bt.autotune(targs)
+    rays = sum(x * y for x, y in sizes)
+
+    def step():
+        bt.trace(targs, graph=True)
+
+    wall = timed(step, ctx.sync, args.steps, args.warmup)
+    kern, st = [], None
+    for _ in range(2):
         step()
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    mrays = W * H * args.steps / wall / 1e6
-
-    kernel_ms = float(np.mean(kern_ms))
-    flops = (steps_fp32 + steps_fp64) * F_STEP_KERR + rays * F_RAY_KERR
-    peak = FP32_PEAK_TFLOPS if prec != L.ST_PREC_FP64 else FP64_PEAK_TFLOPS
-    achieved_tf = flops / (kernel_ms * 1e-3) / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic_kerr09_1080p_fp32.json"
-    if tfile.exists() and prec == L.ST_PREC_FP32 and (W, H) == (1920, 1080):
-        try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except Exception:
-            traffic = None
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(sc)
-
-    line = {
-        "metric": "Mrays/s at 1080p Kerr a=0.9 (plain RK4 vs surrogate+fallback) at <=1e-3 rad error",
-        "value": round(mrays, 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": {"fp32": "f32", "mixed": "f32+f64", "fp64": "f64"}[args.precision],
-        "data": "synthetic (seeded 1024x512 star sky, analytic disc)",
-        "config": {"workload": f"kerr a=0.9 {W}x{H} fixed-fraction RK4 h=0.05r, r_esc=100M, capture 1.01 r+, "
-                               f"thin disc + sky, {args.precision}",
-                   "rays_per_step": W * H, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu-fp32" if prec != L.ST_PREC_FP64 else "valu-fp64",
-                     "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / peak, 4), "traffic": traffic,
-                     "kernel_ms": round(kernel_ms, 4), "flop_per_launch": flops,
-                     "steps_per_ray": round((steps_fp32 + steps_fp64) / max(rays, 1), 3),
-                     "f_step": F_STEP_KERR},
-        "promoted_fraction": round(promoted / max(rays, 1), 5),
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+        st = ctx.stats()
+        kern.append(st["ms_total"])
+    line = header(args, 0, 0, rays, wall,
+                  "kerr a=0.99 3840x2160, 8 frames of an r=15M orbit per launch, volumetric disc h/r=0.05 on a "
+                  f"512x512x32 log-normal grid, ~256 gas samples per crossing, {args.precision}",
+                  {"fp32": "f32", "mixed": "f32+f64"}[args.precision])
+    line["metric"] = "Mrays/s, 4K volumetric Kerr a=0.99, 8 frames per launch"
+    line["config"]["autotuned"] = {"tile": t, "waves_per_cu": w, "ms": round(tune_ms, 3),
+                                   "one_shot": w == L.ST_BATCH_ONE_SHOT}
+    line["roofline"] = roofline(st["steps_fp32"] + st["steps_fp64"], st["rays"], float(np.mean(kern)), F_STEP_KERR,
+                                F_RAY_KERR)
+    line["roofline"]["note"] = "RK4 FLOPs only; gas sampling (trilinear gathers) is not counted"
+    line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
+    line["cpu_baseline"] = None
+    ctx.close()
+    return line
 
 
 def cpu_baseline(sc):
@@ -135,12 +307,11 @@ def cpu_baseline(sc):
         from util import oracle_scene
     except Exception as e:  # the checker is absent: report it, do not guess
         return {"value": None, "unit": "Mrays/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores = min(len(os.sched_getaffinity(0)), 16)
     o = oracle_scene(sc)
     rows = sc.height
     t0 = time.perf_counter()
-    _, steps = o.render("rk4", row0=0, rows=rows, nthreads=cores)
+    o.render("rk4", row0=0, rows=rows, nthreads=cores)
     t_rk4 = time.perf_counter() - t0
     rk4_mrays = sc.width * rows / t_rk4 / 1e6
     trows = max(8, sc.height // 16)
@@ -152,6 +323,31 @@ def cpu_baseline(sc):
     return {"value": round(rk4_mrays, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"full {sc.width}x{rows} frame, FP64 fixed-step RK4 (same algorithm), {t_rk4:.1f}s",
             "tsit5_1e-6": {"value": round(ts_mrays, 5), "sample": f"{trows} centre rows x {sc.width}, {t_ts:.1f}s"}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="kerr09_1080p", choices=["kerr09_1080p", "batch64", "volume4k"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip the mixed / surrogate / D2H legs")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the tracer has no multi-GPU path in this build "
+              "(rays shard by frame/tile with no exchange; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "batch64":
+        line = bench_batch(args)
+    elif args.workload == "volume4k":
+        line = bench_volume(args)
+    else:
+        line = bench_frame(args)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

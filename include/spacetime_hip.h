@@ -176,13 +176,16 @@ int st_batch_create(st_ctx* ctx, int nframes, const st_frame_desc* frames,
                     const double* cam_params64 /* may be NULL */,
                     const float spacetime_params[8]);
 /* Trace every frame of the batch; tile/waves are the launch shape (0 = the
- * autotuned choice for this device). */
+ * autotuned choice for this device).  waves_per_cu > 0 runs a persistent
+ * grid of that many waves per CU over a work queue of wave tiles;
+ * ST_BATCH_ONE_SHOT launches one wave per wave tile instead. */
+#define ST_BATCH_ONE_SHOT (-1)
 int st_trace_batch(st_ctx* ctx, const st_trace_args* args, int tile, int waves_per_cu);
 /* Same, captured once into a hipGraph and replayed on later calls. */
 int st_trace_batch_graph(st_ctx* ctx, const st_trace_args* args, int tile, int waves_per_cu);
 int st_download_batch(st_ctx* ctx, float* host_out, size_t nfloats);
 /* Autotuner for the batched launch: times every (wave tile width in
- * {8, 16, 32, 64}) x (waves per CU in {2, 4, 8, 12, 16}) shape on the
+ * {8, 16, 32, 64}) x (waves per CU in {one-shot, 2, 4, 8, 12, 16}) shape on the
  * current batch, keeps the fastest on the context (used when
  * st_trace_batch gets tile = 0) and, if cache_path is not NULL, records it
  * in that text file under a key of device name, frame count, pixel count,

@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libspacetime_hip.so"
 
 ST_OK, ST_ERR_INVALID_ARG, ST_ERR_OOM, ST_ERR_DEVICE, ST_ERR_LAUNCH, ST_ERR_UNSUPPORTED = range(6)
 ST_PREC_FP32, ST_PREC_MIXED, ST_PREC_FP64 = 0, 1, 2
+ST_BATCH_ONE_SHOT = -1   # st_trace_batch: one wave per wave tile instead of a persistent grid
 REC_N = 12
 REC_FATE, REC_STEPS, REC_DIR, REC_RGB, REC_HAM, REC_REND, REC_FLAGS, REC_RKMIN = 0, 1, 2, 5, 8, 9, 10, 11
 

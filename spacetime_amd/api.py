@@ -375,6 +375,13 @@ class HipPreviewContext:
         L.check(L.lib().st_download(self._h, L.fptr(out)))
         return out
 
+    def download_into(self, out: np.ndarray) -> np.ndarray:
+        """_download_rgb into a caller-owned (H, W, 3) float32 array."""
+        if out.shape != (self.height, self.width, 3) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of shape (height, width, 3)")
+        L.check(L.lib().st_download(self._h, L.fptr(out)))
+        return out
+
     def records(self) -> np.ndarray:
         out = np.empty((self.height, self.width, L.REC_N), dtype=np.float32)
         L.check(L.lib().st_download_records(self._h, L.fptr(out)))

@@ -25,6 +25,9 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    # fuse a*b+c only within one source expression, so every kernel that
+    # inlines the tracer rounds it the same way
+    "-ffp-contract=on",
     # packed FP32 (v_pk_*_f32) stays off: the compiler would otherwise form
     # it from ordinary float code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -313,7 +313,7 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     if (a->precision == ST_PREC_MIXED) {
         ST_HIP(st::launch_tiles(0, kerr, vol, true, D, P, F, a->col0, a->row0, cols, a->rows, tw, c->stream));
         ST_HIP(hipEventRecord(c->ev[1], c->stream));
-        ST_HIP(st::launch_queue64(kerr, vol, D, P, F, nullptr, c->num_cu * 2, c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, D, P, F, nullptr, (size_t)cols * (size_t)a->rows, c->stream));
     } else {
         ST_HIP(st::launch_tiles(a->precision == ST_PREC_FP64, kerr, vol, false, D, P, F, a->col0, a->row0, cols,
                                 a->rows, tw, c->stream));
@@ -454,7 +454,7 @@ int enqueue_batch(st_ctx* c, const st_trace_args* a, int tw, int nblocks) {
     if (route) {
         FrameRef dummy;
         std::memset(&dummy, 0, sizeof(dummy));
-        ST_HIP(st::launch_queue64(kerr, vol, D, P, dummy, c->frames_dev, c->num_cu * 2, c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, D, P, dummy, c->frames_dev, c->queue_cap, c->stream));
     }
     return ST_OK;
 }
@@ -486,18 +486,26 @@ bool tune_lookup(const char* path, const std::string& key, int& tw, int& waves) 
     return found;
 }
 
-void resolve_shape(st_ctx* c, const st_trace_args* a, int& tw, int& waves) {
-    if (tw == 0 || waves <= 0) {
+// waves == ST_BATCH_ONE_SHOT selects the one-shot grid; 0 the tuned shape.
+int resolve_shape(st_ctx* c, const st_trace_args* a, int& tw, int& waves) {
+    if (waves < 0 && waves != ST_BATCH_ONE_SHOT) return fail(ST_ERR_INVALID_ARG, "waves_per_cu must be >= 0 or ST_BATCH_ONE_SHOT");
+    if (tw < 0) return fail(ST_ERR_INVALID_ARG, "tile must be >= 0");
+    if (tw == 0 || waves == 0) {
         int t = c->tuned_tile, w = c->tuned_waves;
         if (!(t && w)) {
             const char* path = std::getenv("ST_TUNE_CACHE");
-            if (!tune_lookup(path, tune_key(c, a), t, w)) { t = 8; w = 8; }
+            if (!tune_lookup(path, tune_key(c, a), t, w)) { t = 8; w = ST_BATCH_ONE_SHOT; }
         }
         if (tw == 0) tw = t;
-        if (waves <= 0) waves = w;
+        if (waves == 0) waves = w;
     }
     if (tw != 8 && tw != 16 && tw != 32 && tw != 64) tw = 8;
     if (waves > 32) waves = 32;
+    return ST_OK;
+}
+
+int batch_blocks(const st_ctx* c, int waves) {
+    return waves == ST_BATCH_ONE_SHOT ? 0 : std::max(1, c->num_cu * waves / 4);
 }
 
 }  // namespace
@@ -508,10 +516,11 @@ int st_trace_batch(st_ctx* c, const st_trace_args* a, int tile, int waves_per_cu
     if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
     int tw = tile, waves = waves_per_cu;
-    resolve_shape(c, a, tw, waves);
-    int rc = prepare_tiles(c, tw);
+    int rc = resolve_shape(c, a, tw, waves);
     if (rc != ST_OK) return rc;
-    const int nblocks = std::max(1, c->num_cu * waves / 4);
+    rc = prepare_tiles(c, tw);
+    if (rc != ST_OK) return rc;
+    const int nblocks = batch_blocks(c, waves);
     ST_HIP(hipEventRecord(c->ev[0], c->stream));
     rc = enqueue_batch(c, a, tw, nblocks);
     if (rc != ST_OK) return rc;
@@ -526,10 +535,11 @@ int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_
     if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
     int tw = tile, waves = waves_per_cu;
-    resolve_shape(c, a, tw, waves);
-    int rc = prepare_tiles(c, tw);
+    int rc = resolve_shape(c, a, tw, waves);
     if (rc != ST_OK) return rc;
-    const int nblocks = std::max(1, c->num_cu * waves / 4);
+    rc = prepare_tiles(c, tw);
+    if (rc != ST_OK) return rc;
+    const int nblocks = batch_blocks(c, waves);
     st::GraphCache& g = c->graph;
     const bool same = g.valid && g.tw == tw && g.nblocks == nblocks && std::memcmp(&g.args, a, sizeof(*a)) == 0;
     if (!same) {
@@ -560,9 +570,9 @@ int st_autotune_batch(st_ctx* c, const st_trace_args* a, const char* cache_path,
     if (!c || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
     if (!c->nframes) return fail(ST_ERR_INVALID_ARG, "no batch: call st_batch_create first");
     static const int tiles[] = {8, 16, 32, 64};
-    static const int waves[] = {2, 4, 8, 12, 16};
+    static const int waves[] = {ST_BATCH_ONE_SHOT, 2, 4, 8, 12, 16};
     float best = 1e30f;
-    int bt = 8, bw = 8;
+    int bt = 8, bw = ST_BATCH_ONE_SHOT;
     for (int tw : tiles) {
         for (int w : waves) {
             int rc = st_trace_batch(c, a, tw, w);   // warm-up (and tile table)

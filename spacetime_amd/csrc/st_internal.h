@@ -86,9 +86,9 @@ int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParam
 hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
                         const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s);
 hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
-                          const FrameRef* frames_dev, int nblocks, hipStream_t s);
+                          const FrameRef* frames_dev, size_t capacity, hipStream_t s);
 hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
-                          int nblocks, hipStream_t s);
+                          size_t capacity, hipStream_t s);
 hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s);
 hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,

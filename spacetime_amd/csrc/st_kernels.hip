@@ -4,10 +4,12 @@
 //                   tile (8x8 by default) so lanes of a wave take similar
 //                   step counts.  FP32 (optionally routing promoted rays to
 //                   the FP64 queue) or FP64.
-//  k_trace_queue64  FP64 re-trace of the compacted promoted-ray queue,
-//                   grid-stride over a device-side count (no host sync).
+//  k_trace_queue    re-trace of a compacted ray queue (FP64 for promoted
+//                   rays, FP32 for the surrogate's fallback); the grid
+//                   covers the queue capacity, the count stays on the device.
 //  k_trace_batch    persistent kernel over a global work queue of pixel
-//                   tiles spanning many frames of uneven size.
+//                   tiles spanning many frames of uneven size;
+//  k_trace_batch_once  the same work items, one wave each, no loop.
 #include <hip/hip_runtime.h>
 
 #include "st_internal.h"
@@ -74,21 +76,33 @@ __global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams 
 }
 
 // Re-trace of queued rays (FP64 for promoted rays, FP32 for the surrogate's
-// fallback).  BATCH: a queue entry's frame index selects frames[]; otherwise
-// every entry belongs to the single frame F0.
+// fallback), one ray per thread.  The grid covers the queue's capacity and
+// blocks past the device-side count return at once, so no host round trip
+// reads the count; without a loop around the tracer the kernel keeps the
+// tile kernel's register footprint.  BATCH: a queue entry's frame index
+// selects frames[]; otherwise every entry belongs to the single frame F0.
 template <typename T, bool KERR, bool VOL, bool BATCH>
 __global__ __launch_bounds__(256) void k_trace_queue(DeviceScene D, TraceParams P, FrameRef F0,
                                                      const FrameRef* frames) {
     const unsigned n = *P.qcount;
+    const unsigned q0 = blockIdx.x * blockDim.x;
+    if (q0 >= n) return;
+    const unsigned q = q0 + threadIdx.x;
     unsigned long long steps = 0;
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    if (q < n) {
         const uint2 e = P.queue[q];
-        const FrameRef& F = BATCH ? frames[e.x] : F0;
-        const int i = (int)(e.y % (unsigned)F.width), j = (int)(e.y / (unsigned)F.width);
         RayOut<T> o;
-        trace_ray<T, KERR, VOL, false>(D, P, F, i, j, o);
-        store_ray(P, F, i, j, o);
-        steps += (unsigned long long)o.steps;
+        if constexpr (BATCH) {
+            const FrameRef& F = frames[e.x];
+            const int i = (int)(e.y % (unsigned)F.width), j = (int)(e.y / (unsigned)F.width);
+            trace_ray<T, KERR, VOL, false>(D, P, F, i, j, o);
+            store_ray(P, F, i, j, o);
+        } else {
+            const int i = (int)(e.y % (unsigned)F0.width), j = (int)(e.y / (unsigned)F0.width);
+            trace_ray<T, KERR, VOL, false>(D, P, F0, i, j, o);
+            store_ray(P, F0, i, j, o);
+        }
+        steps = (unsigned long long)o.steps;
     }
     const unsigned long long tot = wave_sum(steps);
     if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 2], tot);
@@ -102,16 +116,73 @@ __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters,
     if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
 }
 
-// Persistent batched tracer.  Work item w is one 64-pixel wave tile of some
-// frame; tile_base[k] is the first work item of frame k (prefix sum).  Each
-// wave takes items from a global counter until the queue is drained, so
-// frames of different sizes and step counts load-balance across the chip.
+// One wave tile of a batch: work item `item` is one 64-pixel wave tile of
+// some frame; tile_base[k] is the first work item of frame k (prefix sum).
+// `item` is wave-uniform.  Returns the lane's step count; `traced` is set
+// when the lane held a pixel.
+template <bool KERR, bool VOL, bool ROUTE>
+__device__ __forceinline__ unsigned batch_item(const DeviceScene& D, const TraceParams& P, const FrameRef* frames,
+                                               const int* tile_base, int nframes, int tw, unsigned item,
+                                               bool& traced) {
+    const int lane = threadIdx.x & 63;
+    const int th = 64 / tw;
+    // frame of this item: binary search of the prefix sums
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_base[mid] <= (int)item) lo = mid; else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const FrameRef& F = frames[lo];
+    const int local = (int)item - tile_base[lo];
+    const int tiles_x = (F.width + tw - 1) / tw;
+    const int tx = local % tiles_x, ty = local / tiles_x;
+    const int i = tx * tw + lane % tw, j = ty * th + lane / tw;
+    const bool valid = i < F.width && j < F.height;
+    bool push = false;
+    unsigned steps = 0;
+    if (valid) {
+        RayOut<float> o;
+        trace_ray<float, KERR, VOL, ROUTE>(D, P, F, i, j, o);
+        if (ROUTE && o.routed) push = true;
+        else store_ray(P, F, i, j, o);
+        steps = (unsigned)o.steps;
+    }
+    traced = valid;
+    if constexpr (ROUTE) {
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(P.qcount, (unsigned)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (push) {
+                const unsigned off = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+                P.queue[base + off] = make_uint2((unsigned)lo, (unsigned)(j * F.width + i));
+            }
+        }
+    }
+    return steps;
+}
+
+__device__ __forceinline__ void batch_counters(const TraceParams& P, unsigned long long steps,
+                                               unsigned long long nrays) {
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(nrays);
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&P.counters[0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+    }
+}
+
+// Persistent batched tracer: each wave takes wave tiles of any frame from a
+// global counter until the queue is drained, so frames of different sizes
+// and step counts load-balance across the chip with a fixed grid.
 template <bool KERR, bool VOL, bool ROUTE>
 __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene D, TraceParams P, const FrameRef* frames,
                                                      const int* tile_base, int nframes, int total_items,
                                                      int tw, unsigned* work_next) {
     const int lane = threadIdx.x & 63;
-    const int th = 64 / tw;
     unsigned long long steps = 0, nrays = 0;
     for (;;) {
         unsigned item = 0;
@@ -119,47 +190,25 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene D, TraceParams 
         item = __shfl(item, 0, 64);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= (unsigned)total_items) break;
-        // frame of this item: binary search of the prefix sums
-        int lo = 0, hi = nframes - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tile_base[mid] <= (int)item) lo = mid; else hi = mid - 1;
-        }
-        const FrameRef& F = frames[lo];
-        const int local = (int)item - tile_base[lo];
-        const int tiles_x = (F.width + tw - 1) / tw;
-        const int tx = local % tiles_x, ty = local / tiles_x;
-        const int i = tx * tw + lane % tw, j = ty * th + lane / tw;
-        const bool valid = i < F.width && j < F.height;
-        bool push = false;
-        if (valid) {
-            RayOut<float> o;
-            trace_ray<float, KERR, VOL, ROUTE>(D, P, F, i, j, o);
-            if (ROUTE && o.routed) push = true;
-            else store_ray(P, F, i, j, o);
-            steps += (unsigned long long)o.steps;
-            nrays += 1;
-        }
-        if constexpr (ROUTE) {
-            const unsigned long long mask = __ballot(push);
-            if (mask) {
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned base = 0;
-                if (lane == leader) base = atomicAdd(P.qcount, (unsigned)__popcll(mask));
-                base = __shfl(base, leader, 64);
-                if (push) {
-                    const unsigned off = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-                    P.queue[base + off] = make_uint2((unsigned)lo, (unsigned)(j * F.width + i));
-                }
-            }
-        }
+        bool traced;
+        steps += batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced);
+        nrays += traced ? 1u : 0u;
     }
-    const unsigned long long tot = wave_sum(steps);
-    const unsigned long long nv = wave_sum(nrays);
-    if (lane == 0) {
-        if (tot) atomicAdd(&P.counters[0], tot);
-        if (nv) atomicAdd(&P.counters[3], nv);
-    }
+    batch_counters(P, steps, nrays);
+}
+
+// One-shot batched tracer: one wave per wave tile of the whole batch (4 per
+// block); the hardware dispatcher balances the load.  Same work items and
+// results as the persistent form, without the loop around the tracer.
+template <bool KERR, bool VOL, bool ROUTE>
+__global__ __launch_bounds__(256) void k_trace_batch_once(DeviceScene D, TraceParams P, const FrameRef* frames,
+                                                          const int* tile_base, int nframes, int total_items,
+                                                          int tw) {
+    const unsigned item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
+    if (item >= (unsigned)total_items) return;
+    bool traced;
+    const unsigned steps = batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced);
+    batch_counters(P, steps, traced ? 1u : 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -197,7 +246,9 @@ hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceS
 }
 
 hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
-                          const FrameRef* frames_dev, int nblocks, hipStream_t s) {
+                          const FrameRef* frames_dev, size_t capacity, hipStream_t s) {
+    const unsigned nblocks = (unsigned)((capacity + 255) / 256);
+    if (nblocks == 0) return hipSuccess;
 #define ST_Q(KK, VV, BB) \
     hipLaunchKernelGGL((k_trace_queue<double, KK, VV, BB>), dim3(nblocks), dim3(256), 0, s, D, P, F0, frames_dev)
     const bool batch = frames_dev != nullptr;
@@ -213,7 +264,9 @@ hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const Trace
 }
 
 hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
-                          int nblocks, hipStream_t s) {
+                          size_t capacity, hipStream_t s) {
+    const unsigned nblocks = (unsigned)((capacity + 255) / 256);
+    if (nblocks == 0) return hipSuccess;
     if (kerr) {
         if (vol) hipLaunchKernelGGL((k_trace_queue<float, true, true, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
                                     (const FrameRef*)nullptr);
@@ -236,17 +289,35 @@ hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned
 hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,
                         unsigned* work_next, int nblocks, hipStream_t s) {
+    // nblocks > 0: persistent grid of that many blocks; nblocks == 0: one-shot
+    // grid of one wave per work item.
+    if (nblocks > 0) {
 #define ST_B(KK, VV, RR)                                                                                       \
     hipLaunchKernelGGL((k_trace_batch<KK, VV, RR>), dim3(nblocks), dim3(256), 0, s, D, P, frames_dev,          \
                        tile_base_dev, nframes, total_items, tw, work_next)
-    if (kerr) {
-        if (vol) { if (route) ST_B(true, true, true); else ST_B(true, true, false); }
-        else { if (route) ST_B(true, false, true); else ST_B(true, false, false); }
-    } else {
-        if (vol) { if (route) ST_B(false, true, true); else ST_B(false, true, false); }
-        else { if (route) ST_B(false, false, true); else ST_B(false, false, false); }
-    }
+        if (kerr) {
+            if (vol) { if (route) ST_B(true, true, true); else ST_B(true, true, false); }
+            else { if (route) ST_B(true, false, true); else ST_B(true, false, false); }
+        } else {
+            if (vol) { if (route) ST_B(false, true, true); else ST_B(false, true, false); }
+            else { if (route) ST_B(false, false, true); else ST_B(false, false, false); }
+        }
 #undef ST_B
+    } else {
+        const unsigned nb = (unsigned)((total_items + 3) / 4);
+        if (nb == 0) return hipSuccess;
+#define ST_B(KK, VV, RR)                                                                                       \
+    hipLaunchKernelGGL((k_trace_batch_once<KK, VV, RR>), dim3(nb), dim3(256), 0, s, D, P, frames_dev,          \
+                       tile_base_dev, nframes, total_items, tw)
+        if (kerr) {
+            if (vol) { if (route) ST_B(true, true, true); else ST_B(true, true, false); }
+            else { if (route) ST_B(true, false, true); else ST_B(true, false, false); }
+        } else {
+            if (vol) { if (route) ST_B(false, true, true); else ST_B(false, true, false); }
+            else { if (route) ST_B(false, false, true); else ST_B(false, false, false); }
+        }
+#undef ST_B
+    }
     return hipGetLastError();
 }
 

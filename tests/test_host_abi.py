@@ -99,3 +99,20 @@ def test_scene_definitions():
     assert sp[0] == 1.0 and sp[2] == 100.0 and sp[6] == np.float32(0.9)
     px = scenes.sample_pixels(1920, 1080)
     assert px.shape == (4096, 2) and len({(int(i), int(j)) for i, j in px}) == 4096
+
+
+def test_header_constants_match_bindings():
+    """Every ST_* constant the Python bindings define has the header's value."""
+    import re
+    text = (ROOT / "include" / "spacetime_hip.h").read_text()
+    vals = {k: int(v) for k, v in re.findall(r"\b(ST_\w+)\s*=\s*(-?\d+)", text)}
+    vals.update({k: int(v) for k, v in re.findall(r"#define\s+(ST_\w+)\s+\(?(-?\d+)\)?", text)})
+    names = [n for n in dir(L) if n.startswith("ST_") and isinstance(getattr(L, n), int)]
+    assert names
+    for n in names:
+        assert n in vals, n
+        assert getattr(L, n) == vals[n], n
+    rec = {k[len("ST_REC_"):]: v for k, v in vals.items() if k.startswith("ST_REC_")}
+    for k, v in rec.items():
+        if hasattr(L, "REC_" + k):
+            assert getattr(L, "REC_" + k) == v, k
