@@ -109,6 +109,7 @@ typedef struct {
     float ms_fp32;          /* device time of the FP32 (or only) kernel */
     float ms_fp64;          /* device time of the FP64 queue kernel */
     float ms_total;         /* device time of the whole trace call */
+    uint64_t gas_samples;   /* gas density samples taken by the volumetric march */
 } st_stats;
 
 /* Defaults of the reference's render_draft_mtl / _launch_trace! keywords. */

@@ -46,6 +46,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("rays", C.c_uint64), ("steps_fp32", C.c_uint64), ("steps_fp64", C.c_uint64),
         ("promoted", C.c_uint64), ("ms_fp32", C.c_float), ("ms_fp64", C.c_float), ("ms_total", C.c_float),
+        ("gas_samples", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

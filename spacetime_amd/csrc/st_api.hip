@@ -167,7 +167,7 @@ int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, 
     if (e == hipSuccess) e = hipMemset(c->out, 0, sizeof(float) * 3 * (size_t)width * height);
     if (e == hipSuccess) e = hipMalloc(&c->qcount, sizeof(unsigned) * 4);
     if (e == hipSuccess) e = hipMalloc(&c->work_next, sizeof(unsigned) * 4);
-    if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(unsigned long long) * 4);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, sizeof(unsigned long long) * st::kNumCounters);
     if (e == hipSuccess) e = hipMalloc(&c->vol, sizeof(float) * 4);
     if (e == hipSuccess) e = hipMemset(c->vol, 0, sizeof(float) * 4);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
@@ -353,7 +353,7 @@ int st_get_stats(st_ctx* c, st_stats* s) {
     std::memset(s, 0, sizeof(*s));
     if (!c->have_last) return ST_OK;
     ST_HIP(hipStreamSynchronize(c->stream));
-    unsigned long long cnt[4];
+    unsigned long long cnt[st::kNumCounters];
     unsigned q = 0;
     ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
     ST_HIP(hipMemcpy(&q, c->qcount, sizeof(q), hipMemcpyDeviceToHost));
@@ -361,6 +361,7 @@ int st_get_stats(st_ctx* c, st_stats* s) {
     s->steps_fp32 = cnt[0] + cnt[2];
     s->steps_fp64 = cnt[1];
     s->promoted = c->last_mixed ? q : 0;
+    s->gas_samples = cnt[4];
     float ms = 0.0f;
     ST_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     s->ms_fp32 = ms;

@@ -9,6 +9,12 @@
 #include "st_trace.h"
 
 namespace st {
+// Device counters of a trace: 0 FP32 steps, 1 FP64 steps, 2 FP32 fallback
+// steps, 3 rays, 4 gas density samples.
+constexpr int kNumCounters = 5;
+}  // namespace st
+
+namespace st {
 
 std::string& last_error();
 int fail(int code, const std::string& msg);

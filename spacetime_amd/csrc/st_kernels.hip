@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams 
     int i, j;
     bool valid;
     tile_pixel(G, blockIdx.x, threadIdx.x, i, j, valid);
-    unsigned long long steps = 0;
+    unsigned long long steps = 0, gas = 0;
     bool push = false;
     if (valid) {
         RayOut<T> o;
@@ -49,6 +49,7 @@ __global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams 
             push = true;
         } else {
             store_ray(P, F, i, j, o);
+            gas = (unsigned long long)o.gas_samples;
         }
         steps = (unsigned long long)o.steps;
     }
@@ -69,27 +70,33 @@ __global__ __launch_bounds__(256) void k_trace_tiles(DeviceScene D, TraceParams 
     }
     const unsigned long long tot = wave_sum(steps);
     const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
     if ((threadIdx.x & 63) == 0) {
         if (tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 0], tot);
         if (nv) atomicAdd(&P.counters[3], nv);
+        if (ng) atomicAdd(&P.counters[4], ng);
     }
 }
 
 // Re-trace of queued rays (FP64 for promoted rays, FP32 for the surrogate's
-// fallback), one ray per thread.  The grid covers the queue's capacity and
-// blocks past the device-side count return at once, so no host round trip
-// reads the count; without a loop around the tracer the kernel keeps the
-// tile kernel's register footprint.  BATCH: a queue entry's frame index
-// selects frames[]; otherwise every entry belongs to the single frame F0.
-template <typename T, bool KERR, bool VOL, bool BATCH>
+// fallback).  Each wave takes LPW consecutive queue entries, one per lane
+// (lanes >= LPW idle): a short FP64 queue spread over more waves hides more
+// latency than the same rays packed 64 to a wave.  The grid covers the
+// queue's capacity and waves past the device-side count return at once, so
+// no host round trip reads the count; without a loop around the tracer the
+// kernel keeps the tile kernel's register footprint.  BATCH: a queue
+// entry's frame index selects frames[]; otherwise every entry belongs to F0.
+template <typename T, bool KERR, bool VOL, bool BATCH, int LPW>
 __global__ __launch_bounds__(256) void k_trace_queue(DeviceScene D, TraceParams P, FrameRef F0,
                                                      const FrameRef* frames) {
     const unsigned n = *P.qcount;
-    const unsigned q0 = blockIdx.x * blockDim.x;
+    const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned q0 = wave * (unsigned)LPW;
     if (q0 >= n) return;
-    const unsigned q = q0 + threadIdx.x;
-    unsigned long long steps = 0;
-    if (q < n) {
+    const unsigned q = q0 + lane;
+    unsigned long long steps = 0, gas = 0;
+    if (lane < (unsigned)LPW && q < n) {
         const uint2 e = P.queue[q];
         RayOut<T> o;
         if constexpr (BATCH) {
@@ -103,9 +110,12 @@ __global__ __launch_bounds__(256) void k_trace_queue(DeviceScene D, TraceParams 
             store_ray(P, F0, i, j, o);
         }
         steps = (unsigned long long)o.steps;
+        gas = (unsigned long long)o.gas_samples;
     }
     const unsigned long long tot = wave_sum(steps);
-    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 2], tot);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
+    if (lane == 0 && tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 2], tot);
+    if (lane == 0 && ng) atomicAdd(&P.counters[4], ng);
 }
 
 __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters, unsigned* work_next) {
@@ -113,7 +123,7 @@ __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters,
         *qcount = 0u;
         if (work_next) *work_next = 0u;
     }
-    if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
+    if (threadIdx.x < kNumCounters) counters[threadIdx.x] = 0ull;
 }
 
 // One wave tile of a batch: work item `item` is one 64-pixel wave tile of
@@ -123,7 +133,7 @@ __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters,
 template <bool KERR, bool VOL, bool ROUTE>
 __device__ __forceinline__ unsigned batch_item(const DeviceScene& D, const TraceParams& P, const FrameRef* frames,
                                                const int* tile_base, int nframes, int tw, unsigned item,
-                                               bool& traced) {
+                                               bool& traced, unsigned& gas) {
     const int lane = threadIdx.x & 63;
     const int th = 64 / tw;
     // frame of this item: binary search of the prefix sums
@@ -141,11 +151,12 @@ __device__ __forceinline__ unsigned batch_item(const DeviceScene& D, const Trace
     const bool valid = i < F.width && j < F.height;
     bool push = false;
     unsigned steps = 0;
+    gas = 0;
     if (valid) {
         RayOut<float> o;
         trace_ray<float, KERR, VOL, ROUTE>(D, P, F, i, j, o);
         if (ROUTE && o.routed) push = true;
-        else store_ray(P, F, i, j, o);
+        else { store_ray(P, F, i, j, o); gas = (unsigned)o.gas_samples; }
         steps = (unsigned)o.steps;
     }
     traced = valid;
@@ -165,13 +176,16 @@ __device__ __forceinline__ unsigned batch_item(const DeviceScene& D, const Trace
     return steps;
 }
 
+template <bool VOL>
 __device__ __forceinline__ void batch_counters(const TraceParams& P, unsigned long long steps,
-                                               unsigned long long nrays) {
+                                               unsigned long long nrays, unsigned long long gas) {
     const unsigned long long tot = wave_sum(steps);
     const unsigned long long nv = wave_sum(nrays);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
     if ((threadIdx.x & 63) == 0) {
         if (tot) atomicAdd(&P.counters[0], tot);
         if (nv) atomicAdd(&P.counters[3], nv);
+        if (ng) atomicAdd(&P.counters[4], ng);
     }
 }
 
@@ -183,7 +197,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene D, TraceParams 
                                                      const int* tile_base, int nframes, int total_items,
                                                      int tw, unsigned* work_next) {
     const int lane = threadIdx.x & 63;
-    unsigned long long steps = 0, nrays = 0;
+    unsigned long long steps = 0, nrays = 0, ngas = 0;
     for (;;) {
         unsigned item = 0;
         if (lane == 0) item = atomicAdd(work_next, 1u);
@@ -191,10 +205,12 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene D, TraceParams 
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= (unsigned)total_items) break;
         bool traced;
-        steps += batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced);
+        unsigned gas;
+        steps += batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced, gas);
         nrays += traced ? 1u : 0u;
+        ngas += gas;
     }
-    batch_counters(P, steps, nrays);
+    batch_counters<VOL>(P, steps, nrays, ngas);
 }
 
 // One-shot batched tracer: one wave per wave tile of the whole batch (4 per
@@ -207,8 +223,9 @@ __global__ __launch_bounds__(256) void k_trace_batch_once(DeviceScene D, TracePa
     const unsigned item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
     if (item >= (unsigned)total_items) return;
     bool traced;
-    const unsigned steps = batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced);
-    batch_counters(P, steps, traced ? 1u : 0u);
+    unsigned gas;
+    const unsigned steps = batch_item<KERR, VOL, ROUTE>(D, P, frames, tile_base, nframes, tw, item, traced, gas);
+    batch_counters<VOL>(P, steps, traced ? 1u : 0u, gas);
 }
 
 // ---------------------------------------------------------------------------
@@ -245,12 +262,17 @@ hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceS
 #undef ST_DISPATCH
 }
 
+// Lanes per wave of the FP64 queue: promoted rays are a few percent of a
+// frame, too few to give every SIMD more than one full wave.
+constexpr int kLanesFP64 = 16;
+
 hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
                           const FrameRef* frames_dev, size_t capacity, hipStream_t s) {
-    const unsigned nblocks = (unsigned)((capacity + 255) / 256);
+    const unsigned nblocks = (unsigned)((capacity + 4 * kLanesFP64 - 1) / (4 * kLanesFP64));
     if (nblocks == 0) return hipSuccess;
-#define ST_Q(KK, VV, BB) \
-    hipLaunchKernelGGL((k_trace_queue<double, KK, VV, BB>), dim3(nblocks), dim3(256), 0, s, D, P, F0, frames_dev)
+#define ST_Q(KK, VV, BB)                                                                                    \
+    hipLaunchKernelGGL((k_trace_queue<double, KK, VV, BB, kLanesFP64>), dim3(nblocks), dim3(256), 0, s, D, P, \
+                       F0, frames_dev)
     const bool batch = frames_dev != nullptr;
     if (kerr) {
         if (vol) { if (batch) ST_Q(true, true, true); else ST_Q(true, true, false); }
@@ -267,16 +289,17 @@ hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const Trace
                           size_t capacity, hipStream_t s) {
     const unsigned nblocks = (unsigned)((capacity + 255) / 256);
     if (nblocks == 0) return hipSuccess;
+    const FrameRef* none = nullptr;
     if (kerr) {
-        if (vol) hipLaunchKernelGGL((k_trace_queue<float, true, true, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
-                                    (const FrameRef*)nullptr);
-        else hipLaunchKernelGGL((k_trace_queue<float, true, false, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
-                                (const FrameRef*)nullptr);
+        if (vol) hipLaunchKernelGGL((k_trace_queue<float, true, true, false, 64>), dim3(nblocks), dim3(256), 0, s, D,
+                                    P, F0, none);
+        else hipLaunchKernelGGL((k_trace_queue<float, true, false, false, 64>), dim3(nblocks), dim3(256), 0, s, D, P,
+                                F0, none);
     } else {
-        if (vol) hipLaunchKernelGGL((k_trace_queue<float, false, true, false>), dim3(nblocks), dim3(256), 0, s, D, P,
-                                    F0, (const FrameRef*)nullptr);
-        else hipLaunchKernelGGL((k_trace_queue<float, false, false, false>), dim3(nblocks), dim3(256), 0, s, D, P, F0,
-                                (const FrameRef*)nullptr);
+        if (vol) hipLaunchKernelGGL((k_trace_queue<float, false, true, false, 64>), dim3(nblocks), dim3(256), 0, s, D,
+                                    P, F0, none);
+        else hipLaunchKernelGGL((k_trace_queue<float, false, false, false, 64>), dim3(nblocks), dim3(256), 0, s, D,
+                                P, F0, none);
     }
     return hipGetLastError();
 }

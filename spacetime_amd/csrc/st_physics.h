@@ -14,8 +14,9 @@ namespace st {
 // ---------------------------------------------------------------------------
 // Arithmetic traits.  FP32 uses the hardware approximations directly
 // (v_rsq_f32 / v_rcp_f32 / v_sqrt_f32, about 1 ulp); every argument they see
-// on the hot path is a normal number well inside float range.  FP64 uses the
-// correctly rounded library forms.
+// on the hot path is a normal number well inside float range.  FP64 refines
+// the hardware estimates by one Newton step (rsq, rcp, sqrt) and uses the
+// correctly rounded library forms elsewhere (psqrt, exp, ...).
 // ---------------------------------------------------------------------------
 template <typename T> struct Ar;
 template <> struct Ar<float> {
@@ -34,9 +35,25 @@ template <> struct Ar<float> {
     static __device__ __forceinline__ float fmod(float x, float y) { return fmodf(x, y); }
 };
 template <> struct Ar<double> {
-    static __device__ __forceinline__ double rsq(double x) { return 1.0 / ::sqrt(x); }
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    // The hot-path forms start from the hardware estimates (v_rsq_f64,
+    // v_rcp_f64: about 2^-29 relative) and take one Newton step, which
+    // squares the error to FP64 round-off; the IEEE-exact sequences cost
+    // several times as many instructions.
+    static __device__ __forceinline__ double rsq(double x) {
+        const double y = __builtin_amdgcn_rsq(x);
+        const double e = __builtin_fma(-x * y, y, 1.0);
+        return __builtin_fma(0.5 * y, e, y);
+    }
+    static __device__ __forceinline__ double rcp(double x) {
+        const double r = __builtin_amdgcn_rcp(x);
+        return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    }
+    static __device__ __forceinline__ double sqrt(double x) {
+        const double y = rsq(x);
+        const double s = x * y;
+        const double r = __builtin_fma(__builtin_fma(-s, s, x), 0.5 * y, s);
+        return x > 0.0 ? r : x;   // rsq(0) = inf
+    }
     static __device__ __forceinline__ double psqrt(double x) { return ::sqrt(x); }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }

@@ -69,6 +69,7 @@ template <typename T> struct RayOut {
     T dir[3];
     T ham, r_end, rk_min;
     int fate, steps, flags;
+    int gas_samples;   // density samples taken by the volumetric march
     bool routed;
 };
 
@@ -253,19 +254,12 @@ __device__ __noinline__ void shade_disc(const DeviceScene& D, const Kin<T>& K, T
 
 // Arc-length gas march over one accepted step, src/metal.jl:1440-1513.
 template <typename T>
-__device__ __noinline__ void march_gas(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
-                                       T seg, T ds_gas, T& march_acc, T& alpha, T acc[3]) {
+__device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
+                                           T seg, T t0, T ds_gas, int k0, int k1, T& alpha, T acc[3], int& nsamp) {
     using A = Ar<T>;
-    // a non-finite or absurd step length must not spin the sample loop
-    if (!(seg >= T(0) && seg < T(1e6))) return;
-    T t_s = ds_gas - march_acc;
-    if (t_s > seg) {
-        march_acc += seg;
-        return;
-    }
     const T zmax = T(D.vol_params[2]);
-    int guard = 0;
-    while (t_s <= seg && alpha > T(0.003) && ++guard <= 65536) {
+    for (int k = k0; k <= k1 && alpha > T(0.003); ++k) {
+        const T t_s = t0 + T(k) * ds_gas;
         const T fs = t_s / seg;
         const T sx = sp.x + fs * (sn.x - sp.x);
         const T sy = sp.y + fs * (sn.y - sp.y);
@@ -273,6 +267,7 @@ __device__ __noinline__ void march_gas(const DeviceScene& D, const Kin<T>& K, co
         if (A::fabs(sz) < zmax) {
             const T s_cyl = A::psqrt(sx * sx + sy * sy);
             if (s_cyl > T(1e-6)) {
+                ++nsamp;
                 T rho = sample_volume(D, s_cyl, A::atan2(sy, sx), sz);
                 if (rho > T(1e-4)) {
                     const T T_emit = temp_law(D, s_cyl, K.r_isco);
@@ -297,9 +292,44 @@ __device__ __noinline__ void march_gas(const DeviceScene& D, const Kin<T>& K, co
                 }
             }
         }
-        t_s += ds_gas;
     }
-    march_acc = seg - (t_s - ds_gas);
+}
+
+// Arc-length gas march over one accepted step (src/metal.jl:1440-1513).
+// Samples k = 0..n-1 sit at t0 + k ds_gas <= seg, as the reference's running
+// sum places them.  Only those inside the slab |z| < z_max can hold gas, and
+// z is linear along the segment, so the index range of the slab is found
+// directly; the sampler (kept out of line) runs only for segments that
+// touch the slab.
+template <typename T>
+__device__ __forceinline__ void march_gas(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
+                                          T seg, T ds_gas, T& march_acc, T& alpha, T acc[3], int& nsamp) {
+    using A = Ar<T>;
+    // a non-finite or absurd step length must not spin the sample loop
+    if (!(seg >= T(0) && seg < T(1e6))) return;
+    const T t0 = ds_gas - march_acc;   // arc distance to the first sample
+    if (t0 > seg) {
+        march_acc += seg;
+        return;
+    }
+    int n = (int)A::floor((seg - t0) / ds_gas) + 1;
+    if (n > 65536) n = 65536;
+    while (n > 1 && t0 + T(n - 1) * ds_gas > seg) --n;
+    while (n < 65536 && t0 + T(n) * ds_gas <= seg) ++n;
+    const T zmax = T(D.vol_params[2]);
+    int k0 = 0, k1 = n - 1;
+    const T dz = sn.z - sp.z;
+    if (A::fabs(dz) * T(1e6) > zmax) {
+        T ta = (-zmax - sp.z) / dz * seg, tb = (zmax - sp.z) / dz * seg;
+        if (ta > tb) { const T tt = ta; ta = tb; tb = tt; }
+        const T ka = A::floor((ta - t0) / ds_gas), kb = A::floor((tb - t0) / ds_gas) + T(1);
+        k0 = ka > T(0) ? (ka < T(n) ? (int)ka : n) : 0;
+        k1 = kb < T(n - 1) ? (kb > T(-1) ? (int)kb : -1) : n - 1;
+    } else if (!(A::fabs(sp.z) < zmax)) {
+        k1 = -1;
+    }
+    if (k0 <= k1) march_samples(D, K, sp, sn, seg, t0, ds_gas, k0, k1, alpha, acc, nsamp);
+    march_acc = seg - (t0 + T(n - 1) * ds_gas);
 }
 
 // Launch-time radial-potential analysis (src/metal.jl:1008-1094), for a = 0
@@ -384,6 +414,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     const T M = T(P.M), a = KERR ? T(F.has_spin ? F.spin : P.a) : T(0);
     o.routed = false;
     o.steps = 0;
+    o.gas_samples = 0;
     o.flags = (sizeof(T) == 8) ? 1 : 0;
 
     // ---- pose (optional per-pixel shutter, src/metal.jl:833-848) ----
@@ -508,6 +539,8 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
 
     // ---- integration (src/metal.jl:1201-1572) ----
     const bool disc_plane = !VOL && D.disc_on && D.disc_inner > 0.0f && D.disc_outer > D.disc_inner;
+    const T disc_in2 = T(D.disc_inner) * T(D.disc_inner) * T(0.999999);
+    const T disc_out2 = T(D.disc_outer) * T(D.disc_outer) * T(1.000001);
     const T vol_zmax = T(D.vol_params[2]);
     const T vol_s_out = VOL ? A::exp(T(D.vol_params[1])) : T(0);
     const T vol_rb2 = vol_s_out * vol_s_out + vol_zmax * vol_zmax;
@@ -602,14 +635,18 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             const T rn2 = s.x * s.x + s.y * s.y + s.z * s.z;
             if (alpha > T(0.003) && mn(r2, rn2) < vol_rb2) {
                 const T vlen = mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
-                march_gas(D, K, sp, s, h * vlen, ds_gas, march_acc, alpha, acc);
+                march_gas(D, K, sp, s, h * vlen, ds_gas, march_acc, alpha, acc, o.gas_samples);
                 if (alpha < T(0.003)) break;
             }
         }
         if (disc_plane && alpha > T(0.001) && sp.z * s.z < T(0)) {
             const T cfr = sp.z / (sp.z - s.z);
-            shade_disc(D, K, sp.x + cfr * (s.x - sp.x), sp.y + cfr * (s.y - sp.y), sp.px + cfr * (s.px - sp.px),
-                       sp.py + cfr * (s.py - sp.py), alpha, acc);
+            const T xh = sp.x + cfr * (s.x - sp.x), yh = sp.y + cfr * (s.y - sp.y);
+            // only crossings near the annulus reach the out-of-line shader
+            // (which applies the exact inner < s < outer test itself)
+            const T s2 = xh * xh + yh * yh;
+            if (s2 > disc_in2 && s2 < disc_out2)
+                shade_disc(D, K, xh, yh, sp.px + cfr * (s.px - sp.px), sp.py + cfr * (s.py - sp.py), alpha, acc);
         }
     }
     o.steps = steps;
