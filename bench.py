@@ -144,6 +144,17 @@ def bench_frame(args):
             legs["mixed"] = {"value": round(W * H / wm / 1e6, 3), "unit": "Mrays/s",
                              "promoted_fraction": round(sm["promoted"] / max(sm["rays"], 1), 5),
                              "ms_fp32_pass": round(sm["ms_fp32"], 4), "ms_fp64_pass": round(sm["ms_fp64"], 4)}
+            # the reference's adaptive accuracy mode: in-kernel Tsit5, step
+            # capped by the radius rule (ORD 45) at its default tolerance
+            from dataclasses import replace
+            for order in (45, 46):
+                rule_a = replace(sc.rule, order=order, tol=1e-4)
+                aargs = api.trace_args(W, H, rule_a)
+                wa = timed(lambda: ctx.trace(c32, c64, sp, aargs), ctx.sync, max(3, args.steps // 2), 1)
+                ctx.trace(c32, c64, sp, aargs)
+                sa = ctx.stats()
+                legs[f"tsit5_ord{order}_tol1e-4"] = {"value": round(W * H / wa / 1e6, 3), "unit": "Mrays/s",
+                                                      "steps_per_ray": round(sa["steps_fp32"] / max(sa["rays"], 1), 3)}
         if WEIGHTS.exists():
             legs["surrogate_fallback"] = surrogate_leg(args, ctx, sc)
         line["legs"] = legs

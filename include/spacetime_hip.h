@@ -78,14 +78,14 @@ typedef struct {
     int width, height;     /* sensor size (pixels) */
     int nmax;              /* step budget per ray */
     float dt;              /* base affine step */
-    float tol;             /* adaptive tolerance (reserved for ORD 45/46) */
+    float tol;             /* adaptive Tsit5 tolerance (order 45/46) */
     float ju, jv;          /* sub-pixel sample point, or stratum origin if jw > 0 */
     float jw;              /* per-pixel jitter width (0 = pass-wide point) */
     int jseed;             /* per-pixel jitter hash seed */
     float weight;          /* out += weight * sample */
     int row0, rows;        /* row tile [row0, row0 + rows) */
     int col0, cols;        /* column tile; cols < 0 means the full width */
-    int order;             /* 4 RK4, 2 midpoint, 1 Euler */
+    int order;             /* 4 RK4, 2 midpoint, 1 Euler; 45/46 adaptive Tsit5 (src/metal.jl:1278-1381) */
     /* extensions (no counterpart in the reference's launch) */
     float h_floor;         /* lower clamp of h_coef * r / M (reference: 1) */
     int capture_mode;      /* 0 reference kill rules, 1 KS r < capture_factor * r_plus */

@@ -35,6 +35,7 @@ class OraScene(C.Structure):
         ("vol_params", C.c_double * 9), ("gas_ds", C.c_double),
         ("cam", C.c_double * 49), ("width", C.c_int), ("height", C.c_int),
         ("ju", C.c_double), ("jv", C.c_double), ("jw", C.c_double), ("jseed", C.c_int),
+        ("order", C.c_int), ("tol", C.c_double),
     ]
 
 
@@ -184,7 +185,8 @@ class Scene:
                  r_escape=100.0, relativistic=False, dt=1.0, h_coef=0.05, h_floor=0.0, h_cap=1e30,
                  kerr_shrink=False, capture_mode=1, capture_factor=1.01, nmax=6000, esc_interp=True,
                  bardeen=True, disc=None, sky=None, stars=None, volume=None, cam=None, ju=0.5, jv=0.5,
-                 jw=0.0, jseed=0, fisheye_deg=0.0, focus_dist=1.0, equirect=False, gas_ds=0.0):
+                 jw=0.0, jseed=0, fisheye_deg=0.0, focus_dist=1.0, equirect=False, gas_ds=0.0, order=4,
+                 tol=1e-4):
         s = OraScene()
         s.M, s.a = float(M), float(a)
         s.kerr = int(a != 0.0)
@@ -234,6 +236,7 @@ class Scene:
         s.width, s.height = int(width), int(height)
         s.ju, s.jv, s.jw, s.jseed = float(ju), float(jv), float(jw), int(jseed)
         s.gas_ds = float(gas_ds)
+        s.order, s.tol = int(order), float(tol)
         self.s = s
 
     @property

@@ -795,13 +795,33 @@ void ray_start(const ora_scene* sc, int i, int j, ora_ray* o, RayStart& rs) {
 }
 
 
+/* Tsitouras 5(4) tableau.  Coefficients: Ch. Tsitouras, "Runge–Kutta pairs
+ * of order 5(4) satisfying only the first column simplifying assumption",
+ * Comput. Math. Appl. 62 (2011) 770–775 — the same tableau as
+ * src/metal.jl:1317-1358 (there in Float32). */
+const double TC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+const double TA[7][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774, 0}};
+const double TB[7] = {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742,
+                      -3.290069515436081, 2.324710524099774, 0.0};
+const double TBT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                       -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                       0.015151515151515152};
+
 }  // namespace
 
 extern "C" {
 
-/* Fixed-step RK4, the kernel's algorithm in FP64: src/metal.jl:1201-1260
- * (kill/escape/step rule), 1382-1401 (RK4), 1414-1421 (NaN bail),
- * 1440-1571 (gas, thin disc), 1629-1704 (fate, sky). */
+/* The kernel's algorithm in FP64: src/metal.jl:1201-1260 (kill/escape/step
+ * rule), 1382-1401 (RK4; midpoint and Euler for order 2 and 1), 1278-1381
+ * (adaptive Tsit5, order 45/46), 1414-1421 (NaN bail), 1440-1571 (gas, thin
+ * disc), 1629-1704 (fate, sky). */
 int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
     RayStart rs;
     ray_start(sc, i, j, o, rs);
@@ -823,6 +843,11 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
     const int mstep = std::min(std::max((int)sc->vol_params[8], 1), 16);
     const double ds_gas = sc->gas_ds > 0.0 ? sc->gas_ds : mstep * sc->dt;
     int steps = 0;
+    const int order = sc->order == 0 ? 4 : sc->order;
+    const bool adaptive = order == 45 || order == 46;
+    /* adaptive Tsit5 state (src/metal.jl:1190-1197) */
+    double h_carry = -1.0, err_prev = 1e-4, k1_carry[6] = {0, 0, 0, 0, 0, 0};
+    bool k1_valid = false;
     for (int it = 0; it < (rs.no_sky ? 0 : sc->nmax); ++it) {
         const double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
         const double r = std::sqrt(r2);
@@ -853,19 +878,99 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
          * covariant momentum that diverges at finite affine parameter, so a
          * step that straddles the radius must not be completed. */
         double stage_rk2 = 1e300;
-        ora_rhs(sc->kerr, s, rs.p_t, M, sc->a, k1);
-        for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k1[c];
-        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
-        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k2);
-        for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k2[c];
-        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
-        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k3);
-        for (int c = 0; c < 6; ++c) t[c] = s[c] + h * k3[c];
-        stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
-        ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k4);
-        o->rhs_evals += 4;
-        if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap && stage_rk2 < rk2) { hit = true; break; }
-        for (int c = 0; c < 6; ++c) s[c] += (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
+        double h_used = h;
+        if (adaptive && k1_valid) {
+            std::memcpy(k1, k1_carry, sizeof(k1));
+        } else {
+            ora_rhs(sc->kerr, s, rs.p_t, M, sc->a, k1);
+            o->rhs_evals += 1;
+        }
+        if (adaptive) {
+            /* In-kernel adaptive Tsit5, src/metal.jl:1278-1381: ORD 45 caps
+             * the step at the radius rule's h, ORD 46 at 50 dt (approaching
+             * the disc plane no faster than half the distance to its rim);
+             * at most 4 attempts; PI control (Gustafsson) on the accepted
+             * error; the 7th stage is carried as the next k1 (FSAL). */
+            double hmax = h;
+            if (sc->order == 46) {
+                hmax = 50.0 * sc->dt;
+                if (disc_plane) hmax = std::min(hmax, std::max(h, 0.5 * (r - sc->disc_outer)));
+            }
+            double hh = std::min(h_carry > 0.0 ? h_carry : h, hmax);
+            const double pn = std::sqrt(s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+            const double scl = std::max(sc->tol * (1.0 + std::max(r, pn)), 1e-30);
+            double kk[7][6], sn[6];
+            std::memcpy(kk[0], k1, sizeof(k1));
+            bool accepted = false;
+            for (int att = 0; att < 4 && !accepted; ++att) {
+                double smin = 1e300;
+                for (int st = 1; st < 7; ++st) {
+                    /* nested accumulation in stage order, as the kernel does */
+                    for (int c = 0; c < 6; ++c) {
+                        double v = s[c];
+                        for (int m = 0; m < st; ++m) v += (hh * TA[st][m]) * kk[m][c];
+                        t[c] = v;
+                    }
+                    ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, kk[st]);
+                    smin = std::min(smin, ks_r2(sc, t));
+                }
+                for (int c = 0; c < 6; ++c) {
+                    double v = s[c];
+                    for (int m = 0; m < 6; ++m) v += (hh * TB[m]) * kk[m][c];
+                    sn[c] = v;
+                }
+                double k7[6];
+                ora_rhs(sc->kerr, sn, rs.p_t, M, sc->a, k7);
+                smin = std::min(smin, ks_r2(sc, sn));
+                o->rhs_evals += 6;
+                h_used = hh;
+                std::memcpy(k1_carry, k7, sizeof(k7));
+                k1_valid = true;
+                stage_rk2 = smin;
+                double et[6];
+                for (int c = 0; c < 6; ++c) {
+                    double v = TBT[0] * kk[0][c];
+                    for (int m = 1; m < 6; ++m) v += TBT[m] * kk[m][c];
+                    et[c] = v + TBT[6] * k7[c];
+                }
+                const double ex = hh * std::sqrt(et[0] * et[0] + et[1] * et[1] + et[2] * et[2]);
+                const double ep = hh * std::sqrt(et[3] * et[3] + et[4] * et[4] + et[5] * et[5]);
+                const double err = std::max(ex, ep) / scl;
+                if (err <= 1.0 || hh <= 1e-4 * sc->dt) {
+                    accepted = true;
+                    const double g = err > 1e-12 ? 0.9 * std::exp(-0.14 * std::log(err) + 0.08 * std::log(err_prev))
+                                                 : 4.0;
+                    h_carry = std::min(hh * clampd(g, 0.2, 4.0), hmax);
+                    err_prev = std::max(err, 1e-4);
+                } else {
+                    hh = std::max(hh * std::max(0.9 * std::exp(-0.2 * std::log(err)), 0.2), 1e-4 * sc->dt);
+                }
+            }
+            if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap && stage_rk2 < rk2) { hit = true; break; }
+            std::memcpy(s, sn, sizeof(sn));
+        } else if (order == 4) {
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k1[c];
+            stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+            ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k2);
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k2[c];
+            stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+            ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k3);
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + h * k3[c];
+            stage_rk2 = std::min(stage_rk2, ks_r2(sc, t));
+            ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k4);
+            o->rhs_evals += 3;
+            if (sc->capture_mode == 1 && stage_rk2 < rs.rcap * rs.rcap && stage_rk2 < rk2) { hit = true; break; }
+            for (int c = 0; c < 6; ++c) s[c] += (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
+        } else if (order == 2) {
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k1[c];
+            ora_rhs(sc->kerr, t, rs.p_t, M, sc->a, k2);
+            o->rhs_evals += 1;
+            const double rk2m = ks_r2(sc, t);
+            if (sc->capture_mode == 1 && rk2m < rs.rcap * rs.rcap && rk2m < rk2) { hit = true; break; }
+            for (int c = 0; c < 6; ++c) s[c] += h * k2[c];
+        } else {
+            for (int c = 0; c < 6; ++c) s[c] += h * k1[c];
+        }
         ++steps;
         for (int c = 0; c < 3; ++c) { xstart[c] = sp[c]; vstart[c] = k1[c]; }
         have_prev = true;
@@ -873,7 +978,7 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
             !(std::fabs(s[0]) + std::fabs(s[1]) + std::fabs(s[2]) < 1e30)) { hit = true; nonfinite = true; break; }
         const double vlen = std::max(std::sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]), 1e-20);
         if (vol && sh.alpha > 0.003 && std::min(r2, s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) < vol_rb2) {
-            march_gas(sc, sp, s, h * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
+            march_gas(sc, sp, s, h_used * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
             if (sh.alpha < 0.003) break;
         }
         if (disc_plane && sh.alpha > 0.001 && sp[2] * s[2] < 0.0) {
@@ -902,21 +1007,6 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
  * Float32).
  * ------------------------------------------------------------------------- */
 namespace {
-
-const double TC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
-const double TA[7][7] = {
-    {0, 0, 0, 0, 0, 0, 0},
-    {0.161, 0, 0, 0, 0, 0, 0},
-    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
-    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
-    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0, 0},
-    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0, 0},
-    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774, 0}};
-const double TB[7] = {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742,
-                      -3.290069515436081, 2.324710524099774, 0.0};
-const double TBT[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
-                       -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
-                       0.015151515151515152};
 
 struct Tsit {
     const ora_scene* sc;

@@ -75,6 +75,8 @@ typedef struct {
     double ju, jv;         /* sub-pixel sample point (pass stratum origin when jw > 0) */
     double jw;             /* per-pixel hashed jitter width (0 = off) */
     int jseed;             /* per-pixel jitter hash seed */
+    int order;             /* ora_trace_rk4's integrator: 4 RK4, 2 midpoint, 1 Euler, 45/46 adaptive Tsit5 */
+    double tol;            /* adaptive Tsit5 tolerance (order 45/46) */
 } ora_scene;
 
 typedef struct {

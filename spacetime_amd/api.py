@@ -219,7 +219,8 @@ class StepRule:
     nmax: int = 0                    # 0: the reference's budget rule
     esc_interp: bool = False
     bardeen: bool = True
-    order: int = 4
+    order: int = 4                   # 4 RK4, 2 midpoint, 1 Euler, 45/46 adaptive Tsit5
+    tol: float = 1e-4                # adaptive Tsit5 tolerance (render_draft_mtl's `tol`)
 
     @staticmethod
     def reference(dt: float = 0.02) -> "StepRule":
@@ -256,6 +257,7 @@ def trace_args(width: int, height: int, rule: StepRule, *, precision: int = L.ST
     a.rows = height - row0 if rows is None else rows
     a.col0, a.cols = col0, cols
     a.order = rule.order
+    a.tol = rule.tol
     a.h_floor = rule.h_floor
     a.capture_mode = rule.capture_mode
     a.capture_factor = rule.capture_factor
@@ -408,7 +410,7 @@ def render_preview_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, fishey
     """render_preview_mtl (src/metal.jl:1790-1881): one full-frame dispatch of
     centre-of-pixel rays; returns (H, W, 3) float32."""
     if rule is None:
-        rule = StepRule(dt=ctx.dt, r_escape_factor=ctx.r_escape_factor, order=order)
+        rule = StepRule(dt=ctx.dt, r_escape_factor=ctx.r_escape_factor, order=order, tol=tol)
         M = spacetime.M
         r_esc = rule.escape_radius(cam, M)
         rule.nmax = int(min(max(ctx.nmax, math.ceil((75.0 + 6.5 * math.log(r_esc / M)) * M / ctx.dt)), 20000))
@@ -438,7 +440,7 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
     rng = rng if rng is not None else np.random.default_rng()
     if rule is None:
         rule = StepRule.reference(dt)
-    rule = replace(rule, order=order)
+    rule = replace(rule, order=order, tol=tol)
     M = spacetime.M
     c32, c64 = ks_cam_params(cam, M, spin(spacetime), fisheye_deg=fisheye_deg, focus_dist=focus_dist)
     sp = spacetime_params(spacetime, cam, rule, relativistic)

@@ -52,6 +52,7 @@ int upload_float4(float4** dst, const float* src3, int n) {
 
 DeviceScene st::make_scene(const st_ctx* c) {
     DeviceScene D;
+    std::memset(&D, 0, sizeof(D));   // padding too: upload_scene compares bytes
     D.sky = c->sky; D.sky_w = c->sky_w; D.sky_h = c->sky_h;
     D.lut = c->lut; D.lut_n = std::max(c->lut_n, 1);
     D.lut_tmin = c->disc_params[3]; D.lut_tmax = c->disc_params[4];
@@ -64,6 +65,24 @@ DeviceScene st::make_scene(const st_ctx* c) {
     std::memcpy(D.star_params, c->star_params, sizeof(D.star_params));
     if (!c->star_lut) D.star_params[0] = 0.0f;
     return D;
+}
+
+int st::upload_scene(st_ctx* c, const DeviceScene** out) {
+    const DeviceScene D = make_scene(c);
+    if (!c->scene_dev) {
+        ST_HIP(hipMalloc(&c->scene_dev, sizeof(DeviceScene)));
+        c->scene_ok = false;
+    }
+    if (!c->scene_ok || std::memcmp(&D, &c->scene_host, sizeof(D)) != 0) {
+        c->scene_host = D;
+        ST_HIP(hipMemcpyAsync(c->scene_dev, &c->scene_host, sizeof(D), hipMemcpyHostToDevice, c->stream));
+        // the host copy is the source of the transfer: let it land before
+        // scene_host can change again
+        ST_HIP(hipStreamSynchronize(c->stream));
+        c->scene_ok = true;
+    }
+    *out = c->scene_dev;
+    return ST_OK;
 }
 
 namespace {
@@ -79,8 +98,10 @@ int st::fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceP
     if (!(sp[0] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "spacetime_params[0] (M) must be > 0");
     if (std::fabs(sp[6]) > sp[0] * (1.0f + 1e-6f)) return fail(ST_ERR_INVALID_ARG, "|a| must be <= M");
     if (!(a->dt > 0.0f)) return fail(ST_ERR_INVALID_ARG, "dt must be > 0");
-    if (a->order != 4 && a->order != 2 && a->order != 1)
-        return fail(ST_ERR_UNSUPPORTED, "order must be 4, 2 or 1 (adaptive 45/46 not built)");
+    if (a->order != 4 && a->order != 2 && a->order != 1 && a->order != 45 && a->order != 46)
+        return fail(ST_ERR_UNSUPPORTED, "order must be 4, 2, 1 (fixed step) or 45, 46 (adaptive Tsit5)");
+    if ((a->order == 45 || a->order == 46) && !(a->tol > 0.0f))
+        return fail(ST_ERR_INVALID_ARG, "adaptive Tsit5 needs tol > 0");
     if (a->stat) return fail(ST_ERR_UNSUPPORTED, "STAT refinement variant not built");
     if (a->precision < 0 || a->precision > 2) return fail(ST_ERR_INVALID_ARG, "precision must be 0, 1 or 2");
     std::memset(&P, 0, sizeof(P));
@@ -88,7 +109,7 @@ int st::fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceP
     P.h_coef = sp[4]; P.h_cap = sp[5]; P.a = sp[6]; P.rkill2 = sp[7];
     P.nmax = a->nmax > 0 ? a->nmax : ref_nmax(sp[2], sp[0], a->dt);
     P.dt = a->dt; P.ju = a->ju; P.jv = a->jv; P.jw = a->jw; P.jseed = a->jseed;
-    P.weight = a->weight; P.order = a->order; P.h_floor = a->h_floor;
+    P.weight = a->weight; P.order = a->order; P.tol = a->tol; P.h_floor = a->h_floor;
     P.capture_mode = a->capture_mode; P.capture_factor = a->capture_factor;
     P.kerr_shrink = a->kerr_shrink; P.esc_interp = a->esc_interp; P.bardeen = a->bardeen;
     P.promote_band = a->promote_band; P.promote_horizon = a->promote_horizon;
@@ -188,6 +209,7 @@ int st_ctx_destroy(st_ctx* c) {
     free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
     free_dev(c->out); free_dev(c->rec); free_dev(c->queue); free_dev(c->qcount);
     free_dev(c->counters); free_dev(c->work_next); free_dev(c->frames_dev);
+    free_dev(c->scene_dev);
     free_dev(c->batch_out); free_dev(c->tile_base_dev);
     for (int k = 0; k < 3; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -297,7 +319,8 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     int rc = fill_params(c, sp, a, P);
     if (rc != ST_OK) return rc;
     sync_temperature_scale(c, sp[0]);
-    DeviceScene D = make_scene(c);
+    const DeviceScene* D = nullptr;
+    if ((rc = upload_scene(c, &D)) != ST_OK) return rc;
     FrameRef F;
     std::memset(&F, 0, sizeof(F));
     std::memcpy(F.c32, cam, sizeof(F.c32));
@@ -307,15 +330,16 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     F.rec = c->rec_on ? c->rec : nullptr;
     const bool kerr = sp[6] != 0.0f;
     const bool vol = c->has_volume && c->vol_on;
+    const bool adapt = a->order == 45 || a->order == 46;
     ST_HIP(hipEventRecord(c->ev[0], c->stream));
     ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
     const int tw = 8;
     if (a->precision == ST_PREC_MIXED) {
-        ST_HIP(st::launch_tiles(0, kerr, vol, true, D, P, F, a->col0, a->row0, cols, a->rows, tw, c->stream));
+        ST_HIP(st::launch_tiles(0, kerr, vol, true, adapt, D, P, F, a->col0, a->row0, cols, a->rows, tw, c->stream));
         ST_HIP(hipEventRecord(c->ev[1], c->stream));
-        ST_HIP(st::launch_queue64(kerr, vol, D, P, F, nullptr, (size_t)cols * (size_t)a->rows, c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, adapt, D, P, F, nullptr, (size_t)cols * (size_t)a->rows, c->stream));
     } else {
-        ST_HIP(st::launch_tiles(a->precision == ST_PREC_FP64, kerr, vol, false, D, P, F, a->col0, a->row0, cols,
+        ST_HIP(st::launch_tiles(a->precision == ST_PREC_FP64, kerr, vol, false, adapt, D, P, F, a->col0, a->row0, cols,
                                 a->rows, tw, c->stream));
         ST_HIP(hipEventRecord(c->ev[1], c->stream));
     }
@@ -439,12 +463,15 @@ int prepare_tiles(st_ctx* c, int tw) {
 }
 
 int enqueue_batch(st_ctx* c, const st_trace_args* a, int tw, int nblocks) {
+    if (a->order == 45 || a->order == 46)
+        return fail(ST_ERR_UNSUPPORTED, "batched tracing runs the fixed-step integrators (order 4, 2, 1)");
     TraceParams P;
     int rc = fill_params(c, c->batch_st, a, P);
     if (rc != ST_OK) return rc;
     P.assign = 1;
     sync_temperature_scale(c, c->batch_st[0]);
-    DeviceScene D = make_scene(c);
+    const DeviceScene* D = nullptr;
+    if ((rc = upload_scene(c, &D)) != ST_OK) return rc;
     const bool kerr = c->batch_st[6] != 0.0f;
     const bool vol = c->has_volume && c->vol_on;
     const bool route = a->precision == ST_PREC_MIXED;
@@ -455,7 +482,7 @@ int enqueue_batch(st_ctx* c, const st_trace_args* a, int tw, int nblocks) {
     if (route) {
         FrameRef dummy;
         std::memset(&dummy, 0, sizeof(dummy));
-        ST_HIP(st::launch_queue64(kerr, vol, D, P, dummy, c->frames_dev, c->queue_cap, c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, false, D, P, dummy, c->frames_dev, c->queue_cap, c->stream));
     }
     return ST_OK;
 }
@@ -543,6 +570,11 @@ int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_
     const int nblocks = batch_blocks(c, waves);
     st::GraphCache& g = c->graph;
     const bool same = g.valid && g.tw == tw && g.nblocks == nblocks && std::memcmp(&g.args, a, sizeof(*a)) == 0;
+    // the scene block lives in device memory that the graph reads at replay:
+    // bring it up to date outside the capture (an upload synchronizes)
+    sync_temperature_scale(c, c->batch_st[0]);
+    const st::DeviceScene* Dcur = nullptr;
+    if ((rc = upload_scene(c, &Dcur)) != ST_OK) return rc;
     if (!same) {
         if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
         if (g.graph) { (void)hipGraphDestroy(g.graph); g.graph = nullptr; }

@@ -65,6 +65,11 @@ struct st_ctx {
     size_t queue_cap = 0;
     unsigned* qcount = nullptr;
     unsigned long long* counters = nullptr;
+    // the scene block the kernels read through a pointer, re-uploaded when
+    // make_scene's bytes change
+    st::DeviceScene* scene_dev = nullptr;
+    st::DeviceScene scene_host;
+    bool scene_ok = false;
     unsigned* work_next = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool last_mixed = false;
@@ -87,16 +92,24 @@ struct st_ctx {
 namespace st {
 
 DeviceScene make_scene(const st_ctx* c);
+// make_scene(c) in the context's device copy (uploaded when it changed).
+int upload_scene(st_ctx* c, const DeviceScene** out);
 int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P);
 
-hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
-                        const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s);
-hipError_t launch_queue64(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
-                          const FrameRef* frames_dev, size_t capacity, hipStream_t s);
-hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene& D, const TraceParams& P, const FrameRef& F0,
+// adapt: the in-kernel adaptive Tsit5 (order 45/46) instantiations.
+hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, bool adapt, const DeviceScene* D,
+                        const TraceParams& P, const FrameRef& F, int col0, int row0, int cols, int rows, int tw,
+                        hipStream_t s);
+hipError_t launch_queue64(bool kerr, bool vol, bool adapt, const DeviceScene* D, const TraceParams& P,
+                          const FrameRef& F0, const FrameRef* frames_dev, size_t capacity, hipStream_t s);
+hipError_t launch_tiles_adapt(int fp64, bool kerr, bool vol, bool route, const DeviceScene* D, const TraceParams& P,
+                              const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s);
+hipError_t launch_queue64_adapt(bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F0,
+                                const FrameRef* frames_dev, size_t capacity, hipStream_t s);
+hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F0,
                           size_t capacity, hipStream_t s);
 hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s);
-hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene& D, const TraceParams& P,
+hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene* D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,
                         unsigned* work_next, int nblocks, hipStream_t s);
 

@@ -1,0 +1,179 @@
+// Frame and queue trace kernels for gfx950, shared by the translation units
+// that instantiate them (fixed-step RK4/midpoint/Euler in st_kernels.hip,
+// the adaptive Tsit5 forms in st_kernels_adapt.hip).
+//
+//  k_trace_tiles    one ray per lane; each 64-lane wave owns a compact pixel
+//                   tile (8x8 by default) so lanes of a wave take similar
+//                   step counts.  FP32 (optionally routing promoted rays to
+//                   the FP64 queue) or FP64.
+//  k_trace_queue    re-trace of a compacted ray queue (FP64 for promoted
+//                   rays, FP32 for the surrogate's fallback); the grid
+//                   covers the queue capacity, the count stays on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "st_internal.h"
+#include "st_trace.h"
+
+namespace st {
+
+// Tile geometry of a launch: wave tile TW x (64/TW), 4 waves per block
+// stacked vertically.
+struct TileGeom {
+    int col0, row0, cols, rows;
+    int tw;          // wave tile width (8, 16, 32 or 64)
+    int tiles_x;     // block tiles across
+};
+
+__device__ __forceinline__ void tile_pixel(const TileGeom& G, int block, int tid, int& i, int& j, bool& valid) {
+    const int lane = tid & 63, wv = tid >> 6;
+    const int th = 64 / G.tw;               // wave tile height
+    const int bx = block % G.tiles_x, by = block / G.tiles_x;
+    const int li = lane % G.tw, lj = lane / G.tw;
+    i = G.col0 + bx * G.tw + li;
+    j = G.row0 + by * (4 * th) + wv * th + lj;
+    valid = (i < G.col0 + G.cols) && (j < G.row0 + G.rows);
+}
+
+template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT>
+__global__ __launch_bounds__(256) void k_trace_tiles(const DeviceScene* __restrict__ Dp, TraceParams P, FrameRef F, TileGeom G) {
+    const DeviceScene& D = *Dp;
+    int i, j;
+    bool valid;
+    tile_pixel(G, blockIdx.x, threadIdx.x, i, j, valid);
+    unsigned long long steps = 0, gas = 0;
+    bool push = false;
+    if (valid) {
+        RayOut<T> o;
+        trace_ray<T, KERR, VOL, ROUTE, ADAPT>(D, P, F, i, j, o);
+        if (ROUTE && o.routed) {
+            push = true;
+        } else {
+            store_ray(P, F, i, j, o);
+            gas = (unsigned long long)o.gas_samples;
+        }
+        steps = (unsigned long long)o.steps;
+    }
+    if constexpr (ROUTE) {
+        // wave-aggregated append of promoted rays to the FP64 queue
+        const unsigned long long mask = __ballot(push);
+        if (mask) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(P.qcount, (unsigned)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (push) {
+                const unsigned off = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+                P.queue[base + off] = make_uint2(0u, (unsigned)(j * F.width + i));
+            }
+        }
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+        if (ng) atomicAdd(&P.counters[4], ng);
+    }
+}
+
+// Re-trace of queued rays (FP64 for promoted rays, FP32 for the surrogate's
+// fallback).  Each wave takes LPW consecutive queue entries, one per lane
+// (lanes >= LPW idle): a short FP64 queue spread over more waves hides more
+// latency than the same rays packed 64 to a wave.  The grid covers the
+// queue's capacity and waves past the device-side count return at once, so
+// no host round trip reads the count; without a loop around the tracer the
+// kernel keeps the tile kernel's register footprint.  BATCH: a queue
+// entry's frame index selects frames[]; otherwise every entry belongs to F0.
+template <typename T, bool KERR, bool VOL, bool BATCH, int LPW, bool ADAPT>
+__global__ __launch_bounds__(256) void k_trace_queue(const DeviceScene* __restrict__ Dp, TraceParams P, FrameRef F0,
+                                                     const FrameRef* frames) {
+    const DeviceScene& D = *Dp;
+    const unsigned n = *P.qcount;
+    const unsigned wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned q0 = wave * (unsigned)LPW;
+    if (q0 >= n) return;
+    const unsigned q = q0 + lane;
+    unsigned long long steps = 0, gas = 0;
+    if (lane < (unsigned)LPW && q < n) {
+        const uint2 e = P.queue[q];
+        RayOut<T> o;
+        if constexpr (BATCH) {
+            const FrameRef& F = frames[e.x];
+            const int i = (int)(e.y % (unsigned)F.width), j = (int)(e.y / (unsigned)F.width);
+            trace_ray<T, KERR, VOL, false, ADAPT>(D, P, F, i, j, o);
+            store_ray(P, F, i, j, o);
+        } else {
+            const int i = (int)(e.y % (unsigned)F0.width), j = (int)(e.y / (unsigned)F0.width);
+            trace_ray<T, KERR, VOL, false, ADAPT>(D, P, F0, i, j, o);
+            store_ray(P, F0, i, j, o);
+        }
+        steps = (unsigned long long)o.steps;
+        gas = (unsigned long long)o.gas_samples;
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
+    if (lane == 0 && tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 2], tot);
+    if (lane == 0 && ng) atomicAdd(&P.counters[4], ng);
+}
+
+// Lanes per wave of the FP64 queue: promoted rays are a few percent of a
+// frame, too few to give every SIMD more than one full wave.
+constexpr int kLanesFP64 = 16;
+
+template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT>
+hipError_t launch_tiles_t(const DeviceScene* D, const TraceParams& P, const FrameRef& F, const TileGeom& G,
+                          int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_tiles<T, KERR, VOL, ROUTE, ADAPT>), dim3(nblocks), dim3(256), 0, s, D, P, F, G);
+    return hipGetLastError();
+}
+
+template <bool ADAPT>
+hipError_t launch_tiles_impl(int fp64, bool kerr, bool vol, bool route, const DeviceScene* D, const TraceParams& P,
+                             const FrameRef& F, int col0, int row0, int cols, int rows, int tw, hipStream_t s) {
+    TileGeom G;
+    G.col0 = col0; G.row0 = row0; G.cols = cols; G.rows = rows; G.tw = tw;
+    const int th = 64 / tw;
+    G.tiles_x = (cols + tw - 1) / tw;
+    const int tiles_y = (rows + 4 * th - 1) / (4 * th);
+    const int nblocks = G.tiles_x * tiles_y;
+    if (nblocks <= 0) return hipSuccess;
+#define ST_DISPATCH(TT, KK, VV, RR) return launch_tiles_t<TT, KK, VV, RR, ADAPT>(D, P, F, G, nblocks, s)
+    if (fp64) {
+        if (kerr) { if (vol) ST_DISPATCH(double, true, true, false); else ST_DISPATCH(double, true, false, false); }
+        else { if (vol) ST_DISPATCH(double, false, true, false); else ST_DISPATCH(double, false, false, false); }
+    }
+    if (kerr) {
+        if (vol) { if (route) ST_DISPATCH(float, true, true, true); else ST_DISPATCH(float, true, true, false); }
+        else { if (route) ST_DISPATCH(float, true, false, true); else ST_DISPATCH(float, true, false, false); }
+    } else {
+        if (vol) { if (route) ST_DISPATCH(float, false, true, true); else ST_DISPATCH(float, false, true, false); }
+        else { if (route) ST_DISPATCH(float, false, false, true); else ST_DISPATCH(float, false, false, false); }
+    }
+#undef ST_DISPATCH
+}
+
+template <bool ADAPT>
+hipError_t launch_queue64_impl(bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F0,
+                               const FrameRef* frames_dev, size_t capacity, hipStream_t s) {
+    const unsigned nblocks = (unsigned)((capacity + 4 * kLanesFP64 - 1) / (4 * kLanesFP64));
+    if (nblocks == 0) return hipSuccess;
+#define ST_Q(KK, VV, BB)                                                                                      \
+    hipLaunchKernelGGL((k_trace_queue<double, KK, VV, BB, kLanesFP64, ADAPT>), dim3(nblocks), dim3(256), 0, s, \
+                       D, P, F0, frames_dev)
+    const bool batch = frames_dev != nullptr;
+    if (kerr) {
+        if (vol) { if (batch) ST_Q(true, true, true); else ST_Q(true, true, false); }
+        else { if (batch) ST_Q(true, false, true); else ST_Q(true, false, false); }
+    } else {
+        if (vol) { if (batch) ST_Q(false, true, true); else ST_Q(false, true, false); }
+        else { if (batch) ST_Q(false, false, true); else ST_Q(false, false, false); }
+    }
+#undef ST_Q
+    return hipGetLastError();
+}
+
+}  // namespace st

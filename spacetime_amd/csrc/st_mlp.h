@@ -68,12 +68,12 @@ hipError_t launch_adam(float* P, const float* G, float* Mm, float* Vv, size_t n,
                        const float* sumsq, float clip, hipStream_t s);
 hipError_t launch_refresh(const float* P, bf16* W2b, bf16* W2t, bf16* W3b, bf16* W3t, bf16* W4b, bf16* W4t,
                           bf16* W5b, bf16* W5t, hipStream_t s);
-hipError_t launch_make_samples(const DeviceScene& D, const TraceParams& P, const FrameRef* cams, const float* meta,
+hipError_t launch_make_samples(const DeviceScene* D, const TraceParams& P, const FrameRef* cams, const float* meta,
                                int per_cam, int n, size_t total, float* X, float* T, int nblocks, hipStream_t s);
 hipError_t launch_fragment_weights(const float* P, bf16* W2f, bf16* W3f, bf16* W4f, bf16* W5f, hipStream_t s);
 hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const bf16* W4f, const bf16* W5f,
                         const InferView& V, float* pred, int npix, hipStream_t s);
-hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene& D, float* out, float* rec,
+hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene* D, float* out, float* rec,
                          float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s);
 hipError_t launch_gather(const float* X, const float* T, size_t n, uint64_t seed, float* Xb, float* Tb, int B,
                          hipStream_t s);

@@ -300,6 +300,8 @@ int st_mlp_grad(st_mlp* m, const float* x, const float* t, int n, float* y_out, 
 int st_mlp_generate(st_mlp* m, st_ctx* c, const st_surrogate_domain* dom, const float sp[8],
                     const st_trace_args* args) {
     if (!m || !c || !dom || !sp || !args) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (args->order == 45 || args->order == 46)
+        return fail(ST_ERR_UNSUPPORTED, "the surrogate's RK4 paths run the fixed-step integrators");
     const int n = dom->side;
     const int ncam = dom->n_cameras;
     if (n < 8 || ncam < 1 || !(dom->r_max > dom->r_min) || dom->a_max > 0.999f || dom->a_min < 0.0f)
@@ -356,14 +358,20 @@ int st_mlp_generate(st_mlp* m, st_ctx* c, const st_surrogate_domain* dom, const 
     int rc = st::fill_params(c, sp, args, P);
     if (rc != ST_OK) { dfree(fr_dev); dfree(meta_dev); return rc; }
     P.jw = 1.0f; P.ju = 0.0f; P.jv = 0.0f; P.jseed = (int)(dom->seed & 0x7fffffff);
-    st::DeviceScene D = st::make_scene(c);
-    D.disc_on = 0;
-    D.star_params[0] = 0.0f;
+    // training rays see the sky only: a scene block of their own
+    st::DeviceScene Dh = st::make_scene(c);
+    Dh.disc_on = 0;
+    Dh.star_params[0] = 0.0f;
+    st::DeviceScene* D = nullptr;
+    hipError_t e = hipMalloc(&D, sizeof(Dh));
+    if (e == hipSuccess) e = hipMemcpy(D, &Dh, sizeof(Dh), hipMemcpyHostToDevice);
     const int nblocks = c->num_cu * 8;
-    const hipError_t e = launch_make_samples(D, P, fr_dev, meta_dev, per, n, total, m->X, m->T, nblocks, m->stream);
+    if (e == hipSuccess)
+        e = launch_make_samples(D, P, fr_dev, meta_dev, per, n, total, m->X, m->T, nblocks, m->stream);
     const hipError_t e2 = hipStreamSynchronize(m->stream);
     dfree(fr_dev);
     dfree(meta_dev);
+    if (D) (void)hipFree(D);
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(ST_ERR_LAUNCH, std::string("sample generation: ") + hipGetErrorString(e != hipSuccess ? e : e2));
     return ST_OK;
@@ -429,13 +437,16 @@ int st_mlp_render(st_mlp* m, st_ctx* c, const st_surrogate_view* v, const float 
     if (c->has_volume && c->vol_on) return fail(ST_ERR_UNSUPPORTED, "the surrogate has no gas model");
     const int npix = c->width * c->height;
     st::TraceParams P;
+    if (a->order == 45 || a->order == 46)
+        return fail(ST_ERR_UNSUPPORTED, "the surrogate's RK4 paths run the fixed-step integrators");
     int rc = st::fill_params(c, sp, a, P);
     if (rc != ST_OK) return rc;
     InferView V;
     double cam[49];
     if ((rc = make_view(v, sp[6], c->width, c->height, a->ju, a->jv, V, cam)) != ST_OK) return rc;
     if ((rc = prepare_infer(m, c, (size_t)npix)) != ST_OK) return rc;
-    st::DeviceScene D = st::make_scene(c);
+    const st::DeviceScene* D = nullptr;
+    if ((rc = st::upload_scene(c, &D)) != ST_OK) return rc;
     st::FrameRef F;
     std::memset(&F, 0, sizeof(F));
     for (int q = 0; q < 49; ++q) { F.c32[q] = (float)cam[q]; F.c64[q] = cam[q]; }

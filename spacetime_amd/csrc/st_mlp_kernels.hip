@@ -293,9 +293,10 @@ __global__ void k_refresh_bf16(const float* __restrict__ P, bf16* __restrict__ W
 // takes sensor point (i, j) of an n x n grid with hashed jitter; the camera
 // sees directions with tan(alpha), tan(beta) in [-fov, fov].
 // X[s] = features(r, theta, a, alpha, beta); T[s] = (d_cam, fate, valid, n_cam).
-__global__ __launch_bounds__(256) void k_make_samples(DeviceScene D, TraceParams P, const FrameRef* __restrict__ cams,
+__global__ __launch_bounds__(256) void k_make_samples(const DeviceScene* __restrict__ Dp, TraceParams P, const FrameRef* __restrict__ cams,
                                                       const float* __restrict__ meta, int per_cam, int n,
                                                       size_t total, float* __restrict__ X, float* __restrict__ T) {
+    const DeviceScene& D = *Dp;
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += (size_t)gridDim.x * blockDim.x) {
         const int c = (int)(s / per_cam), k = (int)(s % per_cam);
         const int i = k % n, j = k / n;
@@ -492,10 +493,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
 // predicted fate (shadow edge), or when the predicted correction exceeds
 // delta_max (strong lensing near the photon ring).  Fallback pixels are
 // appended to the queue (one atomic per wave); the rest are shaded here.
-__global__ __launch_bounds__(256) void k_surrogate_select(const float* __restrict__ pred, InferView V, DeviceScene D,
+__global__ __launch_bounds__(256) void k_surrogate_select(const float* __restrict__ pred, InferView V, const DeviceScene* __restrict__ Dp,
                                                           float* __restrict__ out, float* __restrict__ rec,
                                                           float weight, uint2* __restrict__ queue,
                                                           unsigned* __restrict__ qcount, int npix) {
+    const DeviceScene& D = *Dp;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     bool push = false;
     if (p < npix) {
@@ -600,7 +602,7 @@ hipError_t launch_refresh(const float* P, bf16* W2b, bf16* W2t, bf16* W3b, bf16*
     return hipGetLastError();
 }
 
-hipError_t launch_make_samples(const DeviceScene& D, const TraceParams& P, const FrameRef* cams, const float* meta,
+hipError_t launch_make_samples(const DeviceScene* D, const TraceParams& P, const FrameRef* cams, const float* meta,
                                int per_cam, int n, size_t total, float* X, float* T, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_make_samples, dim3(nblocks), dim3(256), 0, s, D, P, cams, meta, per_cam, n, total, X, T);
     return hipGetLastError();
@@ -617,7 +619,7 @@ hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const 
     return hipGetLastError();
 }
 
-hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene& D, float* out, float* rec,
+hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene* D, float* out, float* rec,
                          float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s) {
     hipLaunchKernelGGL(k_surrogate_select, dim3((npix + 255) / 256), dim3(256), 0, s, pred, V, D, out, rec, weight,
                        queue, qcount, npix);

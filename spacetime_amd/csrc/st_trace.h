@@ -35,6 +35,7 @@ struct TraceParams {
     int jseed;
     float weight;
     int order;
+    float tol;               // adaptive Tsit5 tolerance (order 45/46)
     float h_floor;
     int capture_mode;
     float capture_factor;
@@ -406,7 +407,7 @@ __device__ __forceinline__ void bardeen(const S6<T>& s, T pt, T M, T a, bool& ca
 // promote_band, that the launch analysis finds captured, or that come within
 // promote_horizon * r_plus of the horizon are handed to the FP64 queue
 // (o.routed) instead of being finished here.
-template <typename T, bool KERR, bool VOL, bool ROUTE>
+template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT = false>
 __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParams& P, const FrameRef& F, int i, int j,
                                           RayOut<T>& o) {
     using A = Ar<T>;
@@ -572,9 +573,21 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     int steps = 0;
     const int order = P.order;
     const int nmax = no_sky ? 0 : P.nmax;
+    // Adaptive Tsit5 state (src/metal.jl:1190-1197): the step carried between
+    // iterations, the previous accepted error for the PI controller, and the
+    // FSAL slope (the 7th stage is the next step's k1).
+    S6<T> k1c = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    bool k1v = false;
+    T h_carry = T(-1), err_prev = T(1e-4);
     for (int it = 0; it < nmax; ++it) {
         T rk2;
-        const S6<T> k1 = rhs<T, KERR>(s, pt, M, a, rk2);
+        S6<T> k1;
+        if (ADAPT && k1v) {
+            k1 = k1c;
+            rk2 = ks_r2<T, KERR>(s.x, s.y, s.z, a);
+        } else {
+            k1 = rhs<T, KERR>(s, pt, M, a, rk2);
+        }
         const T r2 = s.x * s.x + s.y * s.y + s.z * s.z;
         const T r = (sizeof(T) == 4) ? r2 * A::rsq(r2) : A::sqrt(r2);
         if (P.rec) rk_min = mn(rk_min, rk2);
@@ -598,13 +611,88 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             h *= T(0.12) + T(0.88) * fr;
         }
         const S6<T> sp = s;
+        T h_used = h;
         // Capture mode 1 also captures a ray whose RK stage point falls inside
         // the capture radius: below the prograde photon orbit no turning point
         // exists, and a backward ray nearing r_plus has a covariant momentum
         // that diverges at finite affine parameter, so a step straddling the
         // radius must not be completed.
         T rk2d, stage_min;
-        if (order == 4) {
+        if constexpr (ADAPT) {
+            // Adaptive Tsitouras 5(4), src/metal.jl:1278-1381: ORD 45 caps the
+            // step at the radius rule's h; ORD 46 lets it run to 50 dt (and
+            // approach the disc plane no faster than half the distance to its
+            // rim).  At most 4 attempts per step; PI step control.
+            T hmax = h;
+            if (order == 46) {
+                hmax = T(50) * dt;
+                if (disc_plane) hmax = mn(hmax, mx(h, T(0.5) * (r - T(D.disc_outer))));
+            }
+            T hh = mn(h_carry > T(0) ? h_carry : h, hmax);
+            const T pn = A::psqrt(s.px * s.px + s.py * s.py + s.pz * s.pz);
+            const T scl = mx(T(P.tol) * (T(1) + mx(r, pn)), T(1e-30));
+            S6<T> sn = s;
+            bool accepted = false;
+            stage_min = T(1e30);
+            for (int att = 0; att < 4 && !accepted; ++att) {
+                T smin;
+                const S6<T> k2 = rhs<T, KERR>(axpy(s, hh * T(0.161), k1), pt, M, a, rk2d);
+                smin = rk2d;
+                const S6<T> k3 = rhs<T, KERR>(axpy(axpy(s, hh * T(-0.008480655492356989), k1),
+                                                   hh * T(0.335480655492357), k2), pt, M, a, rk2d);
+                smin = mn(smin, rk2d);
+                const S6<T> k4 = rhs<T, KERR>(axpy(axpy(axpy(s, hh * T(2.897153057105493), k1),
+                                                        hh * T(-6.359448489975075), k2),
+                                                   hh * T(4.3622954328695815), k3), pt, M, a, rk2d);
+                smin = mn(smin, rk2d);
+                const S6<T> k5 = rhs<T, KERR>(axpy(axpy(axpy(axpy(s, hh * T(5.325864828439257), k1),
+                                                             hh * T(-11.748883564062828), k2),
+                                                        hh * T(7.4955393428898365), k3),
+                                                   hh * T(-0.09249506636175525), k4), pt, M, a, rk2d);
+                smin = mn(smin, rk2d);
+                const S6<T> k6 = rhs<T, KERR>(axpy(axpy(axpy(axpy(axpy(s, hh * T(5.86145544294642), k1),
+                                                                  hh * T(-12.92096931784711), k2),
+                                                             hh * T(8.159367898576159), k3),
+                                                        hh * T(-0.071584973281401), k4),
+                                                   hh * T(-0.028269050394068383), k5), pt, M, a, rk2d);
+                smin = mn(smin, rk2d);
+                // the 7th stage sits at the 5th-order solution: FSAL
+                sn = axpy(axpy(axpy(axpy(axpy(axpy(s, hh * T(0.09646076681806523), k1), hh * T(0.01), k2),
+                                          hh * T(0.4798896504144996), k3), hh * T(1.379008574103742), k4),
+                               hh * T(-3.290069515436081), k5), hh * T(2.324710524099774), k6);
+                const S6<T> k7 = rhs<T, KERR>(sn, pt, M, a, rk2d);
+                smin = mn(smin, rk2d);
+                h_used = hh;
+                k1c = k7;
+                k1v = true;
+                stage_min = smin;
+                // embedded error: (b - b_hat) over the seven stages
+                S6<T> et = {k1.x * T(-0.00178001105222577714), k1.y * T(-0.00178001105222577714),
+                            k1.z * T(-0.00178001105222577714), k1.px * T(-0.00178001105222577714),
+                            k1.py * T(-0.00178001105222577714), k1.pz * T(-0.00178001105222577714)};
+                et = axpy(et, T(-0.0008164344596567469), k2);
+                et = axpy(et, T(0.007880878010261995), k3);
+                et = axpy(et, T(-0.1447110071732629), k4);
+                et = axpy(et, T(0.5823571654525552), k5);
+                et = axpy(et, T(-0.45808210592918697), k6);
+                et = axpy(et, T(0.015151515151515152), k7);
+                const T ex = hh * A::psqrt(et.x * et.x + et.y * et.y + et.z * et.z);
+                const T ep = hh * A::psqrt(et.px * et.px + et.py * et.py + et.pz * et.pz);
+                const T err = mx(ex, ep) / scl;
+                if (err <= T(1) || hh <= T(1e-4) * dt) {
+                    accepted = true;
+                    const T g = err > T(1e-12)
+                                    ? T(0.9) * A::exp(T(-0.14) * A::log(err) + T(0.08) * A::log(err_prev))
+                                    : T(4);
+                    h_carry = mn(hh * clampv(g, T(0.2), T(4)), hmax);
+                    err_prev = mx(err, T(1e-4));
+                } else {
+                    hh = mx(hh * mx(T(0.9) * A::exp(T(-0.2) * A::log(err)), T(0.2)), T(1e-4) * dt);
+                }
+            }
+            if (P.capture_mode == 1 && stage_min < rcap2 && stage_min < rk2) { hit = true; break; }
+            s = sn;
+        } else if (order == 4) {
             const T hh = T(0.5) * h;
             S6<T> accum = k1;
             S6<T> k = rhs<T, KERR>(axpy(s, hh, k1), pt, M, a, rk2d);
@@ -635,7 +723,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             const T rn2 = s.x * s.x + s.y * s.y + s.z * s.z;
             if (alpha > T(0.003) && mn(r2, rn2) < vol_rb2) {
                 const T vlen = mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
-                march_gas(D, K, sp, s, h * vlen, ds_gas, march_acc, alpha, acc, o.gas_samples);
+                march_gas(D, K, sp, s, h_used * vlen, ds_gas, march_acc, alpha, acc, o.gas_samples);
                 if (alpha < T(0.003)) break;
             }
         }

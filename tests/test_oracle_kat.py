@@ -392,3 +392,29 @@ def test_sim_hash_range_and_determinism(oracle_lib):
     assert np.all(vals * 2 ** 24 == np.round(vals * 2 ** 24))
     assert abs(vals.mean() - 0.5) < 0.02 and abs(vals.std() - math.sqrt(1 / 12)) < 0.02
     assert O.sim_hash(3, 4, 5) == O.sim_hash(3, 4, 5)
+
+
+@pytest.mark.parametrize("order,tol,dir_max", [(45, 1e-6, 2e-6), (45, 1e-8, 2e-6), (46, 1e-8, 2e-5)])
+def test_in_kernel_tsit5_matches_event_located_tsit5(oracle_lib, order, tol, dir_max):
+    """The oracle's restatement of the kernel's adaptive Tsit5 (ORD 45/46:
+    step capped by the radius rule or free to 50 dt, PI control, FSAL) against
+    the oracle's independent Tsit5 with exact event location at tol 1e-11:
+    same fates, escape directions to the escape-interpolation floor, and ORD
+    46 taking fewer steps than ORD 45."""
+    from dataclasses import replace
+    from spacetime_amd import scenes
+    from util import angle_between, oracle_scene
+    sc = scenes.kerr09_1080p(160, 90)
+    rng = np.random.default_rng(11)
+    ij = np.stack([rng.integers(0, sc.width, 300), rng.integers(0, sc.height, 300)], 1).astype(np.int32)
+    truth = oracle_scene(sc).trace_list(ij, "tsit5", 1e-11)
+    sc2 = replace(sc, rule=replace(sc.rule, order=order, tol=tol))
+    r = oracle_scene(sc2).trace_list(ij, "rk4")
+    same = r[:, 0] == truth[:, 0]
+    esc = same & (r[:, 0] == 0)
+    ang = angle_between(r[esc, 2:5], truth[esc, 2:5])
+    assert same.mean() >= 0.99 and esc.sum() > 100
+    assert ang.max() < dir_max, ang.max()
+    if order == 46:
+        r45 = oracle_scene(replace(sc, rule=replace(sc.rule, order=45, tol=tol))).trace_list(ij, "rk4")
+        assert r[:, 1].mean() < r45[:, 1].mean()
