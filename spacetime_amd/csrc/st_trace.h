@@ -116,7 +116,7 @@ __device__ __forceinline__ T sample_volume(const DeviceScene& D, T s, T phi, T z
     using A = Ar<T>;
     const T zmax = T(D.vol_params[2]);
     if (s <= T(0) || A::fabs(z) >= zmax) return T(0);
-    const T ls = A::log(s);
+    const T ls = GasMath<T>::log(s);
     const T ls_in = T(D.vol_params[0]), ls_out = T(D.vol_params[1]);
     if (ls <= ls_in || ls >= ls_out) return T(0);
     const int nr = D.nr, nphi = D.nphi, nz = D.nz;
@@ -269,9 +269,11 @@ __device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K
             const T s_cyl = A::psqrt(sx * sx + sy * sy);
             if (s_cyl > T(1e-6)) {
                 ++nsamp;
-                T rho = sample_volume(D, s_cyl, A::atan2(sy, sx), sz);
+                using G = GasMath<T>;
+                T rho = sample_volume(D, s_cyl, G::atan2(sy, sx), sz);
                 if (rho > T(1e-4)) {
-                    const T T_emit = temp_law(D, s_cyl, K.r_isco);
+                    const T sT = D.T_hold ? mx(s_cyl, K.r_isco) : s_cyl;
+                    const T T_emit = T(D.T0) * G::exp(T(-0.75) * G::log(mx(sT / T(D.Tr0), T(1e-6))));
                     if (s_cyl < K.r_isco) {
                         const T q = s_cyl / K.r_isco;
                         rho *= q * q * q;
@@ -280,11 +282,11 @@ __device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K
                     const T psy = sp.py + fs * (sn.py - sp.py);
                     const T opz = emitter_opz(sx, sy, psx, psy, -K.pt, K.M, K.a, K.r_isco, K.E_isco, K.L_isco);
                     const T T_obs = T_emit * K.scam / opz;
-                    const T inten = T(100) / (A::exp(T(29622.4) / mx(T_obs, T(1))) - T(1));
+                    const T inten = T(100) / (G::exp(T(29622.4) / mx(T_obs, T(1))) - T(1));
                     T col[3];
                     lut_color(D, T_obs, col);
                     const T tau = T(D.vol_params[7]) * rho * ds_gas;
-                    const T aa = T(1) - A::exp(-tau);
+                    const T aa = T(1) - G::exp(-tau);
                     const T w = alpha * aa * inten * T(D.vol_params[6]);
                     acc[0] += w * col[0];
                     acc[1] += w * col[1];

@@ -328,7 +328,17 @@ def test_bad_arguments_are_error_codes(hip):
     with pytest.raises(L.StError):
         ctx.trace(c32, c64, sp, api.trace_args(64, 16, sc.rule))   # size mismatch
     args = api.trace_args(32, 16, sc.rule)
-    args.order = 45
+    args.order = 3                       # no such integrator
+    with pytest.raises(L.StError) as e:
+        ctx.trace(c32, c64, sp, args)
+    assert e.value.code == L.ST_ERR_UNSUPPORTED
+    args = api.trace_args(32, 16, sc.rule)
+    args.order, args.tol = 45, 0.0       # adaptive needs a tolerance
+    with pytest.raises(L.StError) as e:
+        ctx.trace(c32, c64, sp, args)
+    assert e.value.code == L.ST_ERR_INVALID_ARG
+    args = api.trace_args(32, 16, sc.rule)
+    args.stat = 1                        # refinement statistics variant not built
     with pytest.raises(L.StError) as e:
         ctx.trace(c32, c64, sp, args)
     assert e.value.code == L.ST_ERR_UNSUPPORTED
