@@ -192,6 +192,9 @@ int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, 
     if (e == hipSuccess) e = hipMalloc(&c->vol, sizeof(float) * 4);
     if (e == hipSuccess) e = hipMemset(c->vol, 0, sizeof(float) * 4);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_band, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         st_ctx_destroy(c);
         return fail(e == hipErrorOutOfMemory ? ST_ERR_OOM : ST_ERR_DEVICE, hipGetErrorString(e));
@@ -204,6 +207,7 @@ int st_ctx_create(st_ctx** out, const float* bg, int bg_w, int bg_h, int width, 
 int st_ctx_destroy(st_ctx* c) {
     if (!c) return ST_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
@@ -213,6 +217,9 @@ int st_ctx_destroy(st_ctx* c) {
     free_dev(c->batch_out); free_dev(c->tile_base_dev);
     for (int k = 0; k < 3; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->ev_band) (void)hipEventDestroy(c->ev_band);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ST_OK;
@@ -334,7 +341,37 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     ST_HIP(hipEventRecord(c->ev[0], c->stream));
     ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
     const int tw = 8;
-    if (a->precision == ST_PREC_MIXED) {
+    // Mixed policy on frames of at least 128 rows: promoted rays cluster
+    // around the shadow, so the central half of the rows is traced first and
+    // its FP64 queue runs on the second stream while the main stream traces
+    // the two edge bands in FP32 (into their own queue region and counter);
+    // only the edge bands' few promoted rays are left at the end.
+    const int band_rows = 32;   // block height of the 8x8-tile launch
+    int b0 = a->row0 + (a->rows / 4) / band_rows * band_rows;
+    int b1 = a->row0 + (3 * a->rows / 4 + band_rows - 1) / band_rows * band_rows;
+    if (b1 > a->row0 + a->rows) b1 = a->row0 + a->rows;
+    const bool split = a->precision == ST_PREC_MIXED && a->rows >= 4 * band_rows && b1 > b0;
+    if (split) {
+        const size_t n_mid = (size_t)cols * (size_t)(b1 - b0);
+        ST_HIP(st::launch_tiles(0, kerr, vol, true, adapt, D, P, F, a->col0, b0, cols, b1 - b0, tw, c->stream));
+        ST_HIP(hipEventRecord(c->ev_band, c->stream));
+        ST_HIP(hipStreamWaitEvent(c->stream2, c->ev_band, 0));
+        ST_HIP(st::launch_queue64(kerr, vol, adapt, D, P, F, nullptr, n_mid, c->stream2));
+        ST_HIP(hipEventRecord(c->ev_join, c->stream2));
+        TraceParams Pe = P;
+        Pe.queue = c->queue + n_mid;
+        Pe.qcount = c->qcount + 1;
+        if (b0 > a->row0)
+            ST_HIP(st::launch_tiles(0, kerr, vol, true, adapt, D, Pe, F, a->col0, a->row0, cols, b0 - a->row0, tw,
+                                    c->stream));
+        if (b1 < a->row0 + a->rows)
+            ST_HIP(st::launch_tiles(0, kerr, vol, true, adapt, D, Pe, F, a->col0, b1, cols, a->row0 + a->rows - b1, tw,
+                                    c->stream));
+        ST_HIP(hipEventRecord(c->ev[1], c->stream));
+        ST_HIP(st::launch_queue64(kerr, vol, adapt, D, Pe, F, nullptr, (size_t)cols * (size_t)a->rows - n_mid,
+                                  c->stream));
+        ST_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    } else if (a->precision == ST_PREC_MIXED) {
         ST_HIP(st::launch_tiles(0, kerr, vol, true, adapt, D, P, F, a->col0, a->row0, cols, a->rows, tw, c->stream));
         ST_HIP(hipEventRecord(c->ev[1], c->stream));
         ST_HIP(st::launch_queue64(kerr, vol, adapt, D, P, F, nullptr, (size_t)cols * (size_t)a->rows, c->stream));
@@ -378,9 +415,10 @@ int st_get_stats(st_ctx* c, st_stats* s) {
     if (!c->have_last) return ST_OK;
     ST_HIP(hipStreamSynchronize(c->stream));
     unsigned long long cnt[st::kNumCounters];
-    unsigned q = 0;
+    unsigned q2[2] = {0u, 0u};
     ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    ST_HIP(hipMemcpy(&q, c->qcount, sizeof(q), hipMemcpyDeviceToHost));
+    ST_HIP(hipMemcpy(q2, c->qcount, sizeof(q2), hipMemcpyDeviceToHost));
+    const unsigned q = q2[0] + q2[1];
     s->rays = cnt[3];
     s->steps_fp32 = cnt[0] + cnt[2];
     s->steps_fp64 = cnt[1];

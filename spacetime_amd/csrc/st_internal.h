@@ -72,6 +72,11 @@ struct st_ctx {
     bool scene_ok = false;
     unsigned* work_next = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // second stream (default priority) and its hand-over events: the FP64
+    // queue of the frame's central band runs there while the main stream
+    // traces the edge bands in FP32
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_band = nullptr, ev_join = nullptr;
     bool last_mixed = false;
     bool have_last = false;
     // batch

@@ -12,10 +12,8 @@
 namespace st {
 
 __global__ void k_reset_counters(unsigned* qcount, unsigned long long* counters, unsigned* work_next) {
-    if (threadIdx.x == 0) {
-        *qcount = 0u;
-        if (work_next) *work_next = 0u;
-    }
+    if (threadIdx.x < 2) qcount[threadIdx.x] = 0u;   // [0] main queue, [1] edge-band queue
+    if (threadIdx.x == 0 && work_next) *work_next = 0u;
     if (threadIdx.x < kNumCounters) counters[threadIdx.x] = 0ull;
 }
 

@@ -343,7 +343,7 @@ __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ 
 }
 
 // Inference weights in MFMA fragment order: for k-step ks (32 inputs) and
-// output tile nt (16 outputs), lane l's 8 BF16 B-operand values are the
+// output tile nt (16 outputs), lane l's 8 BF16 A-operand values are the
 // contiguous Wf[((ks * ntiles + nt) * 64 + l) * 8 .. +7] = W[nt*16 + l%16][ks*32 + 8*(l/16) ..],
 // so one wave's fragment is a single 1 KiB coalesced load.
 __global__ void k_fragment_weights(const float* __restrict__ P, bf16* __restrict__ W2f, bf16* __restrict__ W3f,
@@ -360,29 +360,54 @@ __global__ void k_fragment_weights(const float* __restrict__ P, bf16* __restrict
     if (rest < 8) {
         const int ks = rest;
         const int o = lane & 15, k = ks * 32 + 8 * (lane >> 4) + e;
-        W5f[idx] = (bf16)(o < NOUT ? P[Layout::W5 + (size_t)o * HID + k] : 0.f);
+        // divided by log2(e): the inference kernel's hidden activations are log2(e) silu(z)
+        W5f[idx] = (bf16)(o < NOUT ? P[Layout::W5 + (size_t)o * HID + k] * 0.6931471805599453f : 0.f);
     }
 }
 
-__device__ __forceinline__ float silu_fast(float z) {
-    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+// SiLU in the log2(e)-scaled space of the inference kernel: for
+// z' = log2(e) z, z' / (1 + 2^-z') = log2(e) silu(z).  One v_exp_f32, one
+// v_rcp_f32 (both about 1 ulp; the result is rounded to BF16).
+__device__ __forceinline__ float silu2(float zs) {
+    return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-zs));
 }
 
-// Fused inference over the pixels of one frame: 128 rays per 256-thread
-// block; activations stay in LDS as BF16 (row stride 264 keeps the 16-byte
-// A-fragment reads conflict-free), each wave computes 64 of the 256 outputs
-// of a layer for all 128 rays with 8 x 4 16x16x32 MFMA tiles.
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Fused inference over the pixels of one frame, 128 rays per 256-thread block
+// (two blocks per CU).  Each hidden layer is computed transposed,
+// C[unit][ray] = W[unit][:] . H[ray][:], with the fragment-ordered weights as
+// the MFMA A operand (global memory, L2-resident) and the BF16 activations as
+// the B operand (LDS, row-major per ray with stride LD = 264 so the 16-byte
+// fragment reads do not conflict).  A lane's accumulator then holds 4
+// consecutive units of one ray and goes back to LDS as one 8-byte store.  The
+// layers run in a log2(e)-scaled space: layer 1 and the hidden biases are
+// scaled by log2(e), the hidden weights are not, and W5 is divided by it, so
+// every SiLU is one exp2 and one reciprocal and the outputs are unscaled.
+// Layer 1 has 3 of its 5 inputs constant over the frame (r, theta, a): its
+// constant part is folded per unit, leaving 2 FMAs per (unit, ray).  Wave w
+// computes units [64w, 64w + 64) of every hidden layer for all 128 rays.
 // pred[p] = (d_world xyz, p_escape, |correction|, unused).
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, const bf16* __restrict__ W2f,
-                                                   const bf16* __restrict__ W3f, const bf16* __restrict__ W4f,
-                                                   const bf16* __restrict__ W5f, InferView V,
-                                                   float* __restrict__ pred, int npix) {
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ P, const bf16* __restrict__ W2f,
+                                                      const bf16* __restrict__ W3f, const bf16* __restrict__ W4f,
+                                                      const bf16* __restrict__ W5f, InferView V,
+                                                      float* __restrict__ pred, int npix) {
     constexpr int ROWS = 128, LD = HID + 8;
-    __shared__ bf16 Hs[ROWS * LD];
-    __shared__ float Xs[ROWS][NIN + 3];
+    constexpr float LOG2E = 1.4426950408889634f;
+    __shared__ __attribute__((aligned(16))) bf16 Hs[ROWS * LD];
+    __shared__ float U[3][HID];       // layer 1, scaled: frame constant, alpha weight, beta weight
+    __shared__ float Xr[ROWS][4];     // per ray: x3, x4, tan(alpha), tan(beta)
     __shared__ float Ys[ROWS][8];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int base = blockIdx.x * ROWS;
+    {
+        float xf[NIN];
+        features(V.r, V.th, V.a, 0.0f, 0.0f, xf);
+        const float* w = P + Layout::W1 + (size_t)tid * NIN;
+        U[0][tid] = LOG2E * (P[Layout::B1 + tid] + w[0] * xf[0] + w[1] * xf[1] + w[2] * xf[2]);
+        U[1][tid] = LOG2E * w[3];
+        U[2][tid] = LOG2E * w[4];
+    }
     if (tid < ROWS) {
         const int p = min(base + tid, npix - 1);
         const int i = p % V.width, j = p / V.width;
@@ -392,86 +417,94 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
         const float tu = u * V.fov, tv = v * V.fov;
         float x[NIN];
         features(V.r, V.th, V.a, atanf(tu), atanf(tv), x);
-#pragma unroll
-        for (int k = 0; k < NIN; ++k) Xs[tid][k] = x[k];
-        Xs[tid][5] = tu;
-        Xs[tid][6] = tv;
+        Xr[tid][0] = x[3];
+        Xr[tid][1] = x[4];
+        Xr[tid][2] = tu;
+        Xr[tid][3] = tv;
     }
     __syncthreads();
-    {   // layer 1 on the vector ALU: thread tid owns hidden unit tid
-        float w[NIN];
+    {   // layer 1 on the vector ALU: thread (ray r, half g) writes units [128g, 128g + 128)
+        const int r = tid & (ROWS - 1), g = tid >> 7;
+        const float xa = Xr[r][0], xb = Xr[r][1];
+#pragma unroll 2
+        for (int c = 0; c < 16; ++c) {
+            const int u0 = 128 * g + 8 * c;
+            bf16x8 hv;
 #pragma unroll
-        for (int k = 0; k < NIN; ++k) w[k] = P[Layout::W1 + tid * NIN + k];
-        const float b = P[Layout::B1 + tid];
-        for (int r = 0; r < ROWS; ++r) {
-            float z = b;
-#pragma unroll
-            for (int k = 0; k < NIN; ++k) z += w[k] * Xs[r][k];
-            Hs[r * LD + tid] = (bf16)silu_fast(z);
+            for (int e = 0; e < 8; ++e)
+                hv[e] = (bf16)silu2(__builtin_fmaf(U[2][u0 + e], xb, __builtin_fmaf(U[1][u0 + e], xa, U[0][u0 + e])));
+            *reinterpret_cast<bf16x8*>(&Hs[r * LD + u0]) = hv;
         }
     }
     __syncthreads();
-    const int lr = lane & 15, lk = 8 * (lane >> 4);
+    const int lr = lane & 15, lg = lane >> 4;
     const bf16* Wl[3] = {W2f, W3f, W4f};
 #pragma unroll 1
     for (int l = 0; l < 3; ++l) {
         const bf16* Wf = Wl[l];
-        f32x4 acc[8][4];
+        const float* bias = P + Layout::B(l + 2);
+        f32x4 acc[4][8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int mt = 0; mt < 4; ++mt) {
+            const int u = wave * 64 + mt * 16 + 4 * lg;
+            const f32x4 b = {LOG2E * bias[u], LOG2E * bias[u + 1], LOG2E * bias[u + 2], LOG2E * bias[u + 3]};
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) acc[i][jj] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
+            for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = b;
+        }
+#pragma unroll 1
         for (int ks = 0; ks < 8; ++ks) {
-            bf16x8 bq[4];
+            bf16x8 aq[4];
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-                bq[jj] = *reinterpret_cast<const bf16x8*>(Wf + (((size_t)ks * 16 + wave * 4 + jj) * 64 + lane) * 8);
+            for (int mt = 0; mt < 4; ++mt)
+                aq[mt] = *reinterpret_cast<const bf16x8*>(Wf + (((size_t)ks * 16 + wave * 4 + mt) * 64 + lane) * 8);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const bf16x8 aq = *reinterpret_cast<const bf16x8*>(&Hs[(i * 16 + lr) * LD + ks * 32 + lk]);
+            for (int nt = 0; nt < 8; ++nt) {
+                const bf16x8 bq = *reinterpret_cast<const bf16x8*>(&Hs[(nt * 16 + lr) * LD + ks * 32 + 8 * lg]);
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj)
-                    acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq, bq[jj], acc[i][jj], 0, 0, 0);
+                for (int mt = 0; mt < 4; ++mt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[mt], bq, acc[mt][nt], 0, 0, 0);
             }
         }
-        __syncthreads();
-        const float* bias = P + Layout::B(l + 2);
+        __syncthreads();   // every wave has read this layer's input
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int col = wave * 64 + jj * 16 + lr;
-            const float bb = bias[col];
+        for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int nt = 0; nt < 8; ++nt) {
+                bf16x4 hv;
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    Hs[(i * 16 + 4 * (lane >> 4) + v) * LD + col] = (bf16)silu_fast(acc[i][jj][v] + bb);
-        }
+                for (int v = 0; v < 4; ++v) hv[v] = (bf16)silu2(acc[mt][nt][v]);
+                *reinterpret_cast<bf16x4*>(&Hs[(nt * 16 + lr) * LD + wave * 64 + mt * 16 + 4 * lg]) = hv;
+            }
         __syncthreads();
     }
-    {   // output layer: wave w takes row tiles 2w, 2w+1 of the single 16-wide tile
-        f32x4 acc5[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    {   // output layer (unit tile 0: 5 outputs + zero padding); wave w takes ray tiles 2w, 2w+1
+        f32x4 acc5[2];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int u = 4 * lg + v;
+            const float bb = u < NOUT ? P[Layout::B5 + u] : 0.0f;
+            acc5[0][v] = bb;
+            acc5[1][v] = bb;
+        }
 #pragma unroll
         for (int ks = 0; ks < 8; ++ks) {
-            const bf16x8 bq = *reinterpret_cast<const bf16x8*>(W5f + ((size_t)ks * 64 + lane) * 8);
+            const bf16x8 aq = *reinterpret_cast<const bf16x8*>(W5f + ((size_t)ks * 64 + lane) * 8);
 #pragma unroll
             for (int ii = 0; ii < 2; ++ii) {
-                const bf16x8 aq =
-                    *reinterpret_cast<const bf16x8*>(&Hs[((2 * wave + ii) * 16 + lr) * LD + ks * 32 + lk]);
+                const bf16x8 bq =
+                    *reinterpret_cast<const bf16x8*>(&Hs[((2 * wave + ii) * 16 + lr) * LD + ks * 32 + 8 * lg]);
                 acc5[ii] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq, bq, acc5[ii], 0, 0, 0);
             }
         }
-        if (lr < NOUT) {
-            const float bb = P[Layout::B5 + lr];
 #pragma unroll
-            for (int ii = 0; ii < 2; ++ii)
+        for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) Ys[(2 * wave + ii) * 16 + 4 * (lane >> 4) + v][lr] = acc5[ii][v] + bb;
-        }
+            for (int v = 0; v < 4; ++v)
+                if (4 * lg + v < NOUT) Ys[(2 * wave + ii) * 16 + lr][4 * lg + v] = acc5[ii][v];
     }
     __syncthreads();
     if (tid < ROWS && base + tid < npix) {
-        const float tu = Xs[tid][5], tv = Xs[tid][6];
+        const float tu = Xr[tid][2], tv = Xr[tid][3];
         const float inu = 1.0f / sqrtf(tu * tu + tv * tv + 1.0f);
         const float dx = Ys[tid][0], dy = Ys[tid][1], dz = Ys[tid][2];
         float cx = tu * inu + dx, cy = tv * inu + dy, cz = inu + dz;
