@@ -360,16 +360,15 @@ __global__ void k_fragment_weights(const float* __restrict__ P, bf16* __restrict
     if (rest < 8) {
         const int ks = rest;
         const int o = lane & 15, k = ks * 32 + 8 * (lane >> 4) + e;
-        // divided by log2(e): the inference kernel's hidden activations are log2(e) silu(z)
-        W5f[idx] = (bf16)(o < NOUT ? P[Layout::W5 + (size_t)o * HID + k] * 0.6931471805599453f : 0.f);
+        W5f[idx] = (bf16)(o < NOUT ? P[Layout::W5 + (size_t)o * HID + k] : 0.f);
     }
 }
 
-// SiLU in the log2(e)-scaled space of the inference kernel: for
-// z' = log2(e) z, z' / (1 + 2^-z') = log2(e) silu(z).  One v_exp_f32, one
-// v_rcp_f32 (both about 1 ulp; the result is rounded to BF16).
-__device__ __forceinline__ float silu2(float zs) {
-    return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-zs));
+// SiLU for the inference kernel: z / (1 + 2^(-z log2 e)) with one v_exp_f32
+// and one v_rcp_f32 (both about 1 ulp; the result is rounded to BF16, as the
+// training forward pass rounds it).
+__device__ __forceinline__ float silu2(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
 }
 
 typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -380,10 +379,9 @@ typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 // the MFMA A operand (global memory, L2-resident) and the BF16 activations as
 // the B operand (LDS, row-major per ray with stride LD = 264 so the 16-byte
 // fragment reads do not conflict).  A lane's accumulator then holds 4
-// consecutive units of one ray and goes back to LDS as one 8-byte store.  The
-// layers run in a log2(e)-scaled space: layer 1 and the hidden biases are
-// scaled by log2(e), the hidden weights are not, and W5 is divided by it, so
-// every SiLU is one exp2 and one reciprocal and the outputs are unscaled.
+// consecutive units of one ray and goes back to LDS as one 8-byte store.
+// The bias is added after the accumulation and activations are rounded to
+// BF16 at the same points as in the training forward pass.
 // Layer 1 has 3 of its 5 inputs constant over the frame (r, theta, a): its
 // constant part is folded per unit, leaving 2 FMAs per (unit, ray).  Wave w
 // computes units [64w, 64w + 64) of every hidden layer for all 128 rays.
@@ -393,9 +391,8 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
                                                       const bf16* __restrict__ W5f, InferView V,
                                                       float* __restrict__ pred, int npix) {
     constexpr int ROWS = 128, LD = HID + 8;
-    constexpr float LOG2E = 1.4426950408889634f;
     __shared__ __attribute__((aligned(16))) bf16 Hs[ROWS * LD];
-    __shared__ float U[3][HID];       // layer 1, scaled: frame constant, alpha weight, beta weight
+    __shared__ float U[3][HID];       // layer 1: frame-constant part, alpha weight, beta weight
     __shared__ float Xr[ROWS][4];     // per ray: x3, x4, tan(alpha), tan(beta)
     __shared__ float Ys[ROWS][8];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -404,9 +401,9 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
         float xf[NIN];
         features(V.r, V.th, V.a, 0.0f, 0.0f, xf);
         const float* w = P + Layout::W1 + (size_t)tid * NIN;
-        U[0][tid] = LOG2E * (P[Layout::B1 + tid] + w[0] * xf[0] + w[1] * xf[1] + w[2] * xf[2]);
-        U[1][tid] = LOG2E * w[3];
-        U[2][tid] = LOG2E * w[4];
+        U[0][tid] = P[Layout::B1 + tid] + w[0] * xf[0] + w[1] * xf[1] + w[2] * xf[2];
+        U[1][tid] = w[3];
+        U[2][tid] = w[4];
     }
     if (tid < ROWS) {
         const int p = min(base + tid, npix - 1);
@@ -445,12 +442,9 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
         const float* bias = P + Layout::B(l + 2);
         f32x4 acc[4][8];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            const int u = wave * 64 + mt * 16 + 4 * lg;
-            const f32x4 b = {LOG2E * bias[u], LOG2E * bias[u + 1], LOG2E * bias[u + 2], LOG2E * bias[u + 3]};
+        for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = b;
-        }
+            for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int ks = 0; ks < 8; ++ks) {
             bf16x8 aq[4];
@@ -467,25 +461,21 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
         }
         __syncthreads();   // every wave has read this layer's input
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+            const int u = wave * 64 + mt * 16 + 4 * lg;
+            const float b[4] = {bias[u], bias[u + 1], bias[u + 2], bias[u + 3]};
 #pragma unroll
             for (int nt = 0; nt < 8; ++nt) {
                 bf16x4 hv;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) hv[v] = (bf16)silu2(acc[mt][nt][v]);
+                for (int v = 0; v < 4; ++v) hv[v] = (bf16)silu2(acc[mt][nt][v] + b[v]);
                 *reinterpret_cast<bf16x4*>(&Hs[(nt * 16 + lr) * LD + wave * 64 + mt * 16 + 4 * lg]) = hv;
             }
+        }
         __syncthreads();
     }
     {   // output layer (unit tile 0: 5 outputs + zero padding); wave w takes ray tiles 2w, 2w+1
-        f32x4 acc5[2];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int u = 4 * lg + v;
-            const float bb = u < NOUT ? P[Layout::B5 + u] : 0.0f;
-            acc5[0][v] = bb;
-            acc5[1][v] = bb;
-        }
+        f32x4 acc5[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int ks = 0; ks < 8; ++ks) {
             const bf16x8 aq = *reinterpret_cast<const bf16x8*>(W5f + ((size_t)ks * 64 + lane) * 8);
@@ -500,7 +490,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
         for (int ii = 0; ii < 2; ++ii)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                if (4 * lg + v < NOUT) Ys[(2 * wave + ii) * 16 + lr][4 * lg + v] = acc5[ii][v];
+                if (4 * lg + v < NOUT) Ys[(2 * wave + ii) * 16 + lr][4 * lg + v] = acc5[ii][v] + P[Layout::B5 + 4 * lg + v];
     }
     __syncthreads();
     if (tid < ROWS && base + tid < npix) {

@@ -9,6 +9,7 @@ also its fallback and the judge of its error.
 from __future__ import annotations
 
 import ctypes as C
+from pathlib import Path
 from typing import Optional
 
 import numpy as np
@@ -16,8 +17,19 @@ import numpy as np
 from . import _lib as L
 from . import api
 
+# Weights trained by `python -m spacetime_amd.train_surrogate` (FP32 master
+# copy in Layout order; st_mlp_param_count() floats).
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "surrogate_weights.f32"
+
 
 class Surrogate:
+    @classmethod
+    def pretrained(cls, path: Optional[Path] = None) -> "Surrogate":
+        """The network with the package's trained weights."""
+        s = cls(seed=1)
+        s.set_params(np.fromfile(path or DEFAULT_WEIGHTS, dtype=np.float32))
+        return s
+
     def __init__(self, seed: int = 1):
         h = C.c_void_p()
         L.check(L.lib().st_mlp_create(C.byref(h), int(seed)))

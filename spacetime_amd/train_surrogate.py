@@ -19,8 +19,7 @@ from pathlib import Path
 import numpy as np
 
 from . import api, scenes, surrogate
-
-DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "surrogate_weights.f32"
+from .surrogate import DEFAULT_WEIGHTS
 
 
 def main(argv=None):

@@ -124,3 +124,64 @@ def test_partial_fallback_consistency(hip, trained):
     # fallback pixels are the RK4 pixels
     rel = np.abs(sur[~surr] - plain[~surr]).max(axis=1) / np.maximum(np.abs(plain[~surr]).max(axis=1), 1e-6)
     assert np.mean(rel <= 1e-6) >= 0.999 and rel.max() <= 1e-3
+
+
+WEIGHTS = surrogate.DEFAULT_WEIGHTS
+
+
+def _gate_stats(s, W=640, H=360, conf=0.99, delta_max=1.0, edge_px=2):
+    """Surrogate + fallback frame against the plain RK4 frame (disc off)."""
+    import time
+    sc = scenes.kerr09_1080p(W, H)
+    sc.disc = api.AccretionDisc(inner_radius=0.0, outer_radius=0.0)
+    ctx = api.HipPreviewContext(scenes.sky(), W, H)
+    c32, c64 = sc.cam_params()
+    sp = sc.spacetime_params()
+    args = api.trace_args(W, H, sc.rule)
+    ctx.enable_records(True)
+    ctx.clear()
+    ctx.trace(c32, c64, sp, args)
+    prec = ctx.records().reshape(-1, L.REC_N).copy()
+    ctx.clear()
+    s.render(ctx, POS, FOV, sp, args, conf=conf, delta_max=delta_max, edge_px=edge_px)
+    srec = ctx.records().reshape(-1, L.REC_N).copy()
+    ctx.enable_records(False)
+    t = {}
+    for name, fn in (("plain", lambda: ctx.trace(c32, c64, sp, args)),
+                     ("surrogate", lambda: s.render(ctx, POS, FOV, sp, args, conf=conf, delta_max=delta_max,
+                                                    edge_px=edge_px))):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            fn()
+        ctx.sync()
+        t[name] = (time.perf_counter() - t0) / 10
+    ctx.close()
+    kept = (srec[:, L.REC_FLAGS].astype(int) & 8) != 0
+    both = (prec[:, L.REC_FATE] == 0) & (srec[:, L.REC_FATE] == 0)
+    d1 = prec[both][:, L.REC_DIR:L.REC_DIR + 3].astype(np.float64)
+    d2 = srec[both][:, L.REC_DIR:L.REC_DIR + 3].astype(np.float64)
+    ang = np.arccos(np.clip(np.sum(d1 * d2, axis=1) / np.linalg.norm(d1, axis=1) / np.linalg.norm(d2, axis=1), -1, 1))
+    return dict(kept=float(kept.mean()), fate=float(np.mean(prec[:, L.REC_FATE] == srec[:, L.REC_FATE])),
+                p50=float(np.median(ang)), p99=float(np.percentile(ang, 99)), ms_plain=1e3 * t["plain"],
+                ms_surrogate=1e3 * t["surrogate"])
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="no trained surrogate weights in the package")
+def test_trained_network_takes_pixels_off_rk4(hip):
+    """What the trained network gains over the method without it: at the
+    preview gate it answers most pixels itself, agreeing with RK4 on the
+    fate of >= 99.9% of them, where an untrained network of the same shape
+    hands almost everything back to RK4 (or gets the fates wrong)."""
+    trained = surrogate.Surrogate.pretrained()
+    print("case: trained surrogate at the preview gate, 640x360", flush=True)
+    g = _gate_stats(trained)
+    print(f"trained: {g}")
+    fresh = surrogate.Surrogate(seed=1)
+    u = _gate_stats(fresh)
+    print(f"untrained: {u}")
+    trained.close()
+    fresh.close()
+    assert g["kept"] >= 0.9 and g["fate"] >= 0.999 and g["p50"] <= 2e-3
+    assert not (u["kept"] >= 0.9 and u["fate"] >= 0.999 and u["p50"] <= 2e-3)
