@@ -435,28 +435,40 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
     }
     __syncthreads();
     const int lr = lane & 15, lg = lane >> 4;
-    const bf16* Wl[3] = {W2f, W3f, W4f};
 #pragma unroll 1
     for (int l = 0; l < 3; ++l) {
-        const bf16* Wf = Wl[l];
+        const bf16* __restrict__ Wf = l == 0 ? W2f : (l == 1 ? W3f : W4f);
         const float* bias = P + Layout::B(l + 2);
+        // this wave's A fragments: k-step ks, unit tile (wave * 4 + mt)
+        const bf16* wq = Wf + ((size_t)(wave * 4) * 64 + lane) * 8;
+        constexpr size_t KSTEP = (size_t)16 * 64 * 8;   // bf16 per k-step over all 16 unit tiles
         f32x4 acc[4][8];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int ks = 0; ks < 8; ++ks) {
-            bf16x8 aq[4];
+        // A fragments double-buffered one k-step ahead so the L2 latency of
+        // step ks + 1 hides behind the 32 MFMAs of step ks
+        bf16x8 aq[2][4];
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                aq[mt] = *reinterpret_cast<const bf16x8*>(Wf + (((size_t)ks * 16 + wave * 4 + mt) * 64 + lane) * 8);
+        for (int mt = 0; mt < 4; ++mt) aq[0][mt] = *reinterpret_cast<const bf16x8*>(wq + (size_t)mt * 64 * 8);
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            if (ks + 1 < 8) {
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+                    aq[(ks + 1) & 1][mt] =
+                        *reinterpret_cast<const bf16x8*>(wq + (size_t)(ks + 1) * KSTEP + (size_t)mt * 64 * 8);
+            }
+            // keep the prefetch ahead of this step's MFMAs (the scheduler would
+            // otherwise sink it next to its first use)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int nt = 0; nt < 8; ++nt) {
                 const bf16x8 bq = *reinterpret_cast<const bf16x8*>(&Hs[(nt * 16 + lr) * LD + ks * 32 + 8 * lg]);
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[mt], bq, acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[ks & 1][mt], bq, acc[mt][nt], 0, 0, 0);
             }
         }
         __syncthreads();   // every wave has read this layer's input
