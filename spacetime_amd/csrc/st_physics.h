@@ -102,6 +102,19 @@ template <> struct GasMath<double> {
     static __device__ __forceinline__ double atan2(double y, double x) { return ::atan2(y, x); }
 };
 
+// Division and square root of the per-sample gas shading: in FP32 the
+// hardware reciprocal and square root (about 1 ulp) in place of the
+// correctly rounded sequences; in FP64 the exact operations.
+template <typename T> struct Fast;
+template <> struct Fast<float> {
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+template <> struct Fast<double> {
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+};
+
 template <typename T> __device__ __forceinline__ T mn(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T mx(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T clampv(T x, T lo, T hi) { return mn(mx(x, lo), hi); }
@@ -255,38 +268,43 @@ __device__ __forceinline__ void isco(T M, T a, T& r_isco, T& E_isco, T& L_isco) 
     L_isco = (T(2) * M / (T(3) * T(1.7320508075688772))) * (T(1) + T(2) * A::psqrt(T(3) * r_isco / M - T(2)));
 }
 
-// Emitter shift -p.u/E (src/metal.jl:396-428).
-template <typename T>
+// Emitter shift -p.u/E (src/metal.jl:396-428).  FAST: FP32 divisions and
+// square roots through the hardware reciprocal / square root (the gas
+// sampler, once per density sample); the thin disc keeps the exact forms.
+template <typename T, bool FAST = false>
 __device__ __forceinline__ T emitter_opz(T sx, T sy, T psx, T psy, T E, T M, T a, T r_isco, T E_isco, T L_isco) {
     using A = Ar<T>;
+    auto dv = [](T x, T y) { return FAST ? Fast<T>::div(x, y) : x / y; };
+    auto sq = [](T x) { return FAST ? Fast<T>::sqrt(x) : A::psqrt(x); };
     const T a2 = a * a;
     const T s2 = sx * sx + sy * sy;
-    const T r = A::psqrt(mx(s2 - a2, T(1e-6)));
-    const T sqM = A::psqrt(M), sqr = A::psqrt(r), r32 = r * sqr;
+    const T r = sq(mx(s2 - a2, T(1e-6)));
+    const T sqM = sq(M), sqr = sq(r), r32 = r * sqr;
     const T Dl = r * r - T(2) * M * r + a2;
     T ut, uphi, ur = T(0);
     if (r >= r_isco) {
         const T den = r32 - T(3) * M * sqr + T(2) * a * sqM;
-        ut = (r32 + a * sqM) / (A::psqrt(r32) * A::psqrt(mx(den, T(1e-2))));
-        uphi = sqM / (r32 + a * sqM) * ut;
+        ut = dv(r32 + a * sqM, sq(r32) * sq(mx(den, T(1e-2))));
+        uphi = dv(sqM, r32 + a * sqM) * ut;
     } else {
-        const T inv_D = T(1) / mx(Dl, T(1e-3));
-        ut = ((r * r + a2 + T(2) * M * a2 / r) * E_isco - T(2) * M * a * L_isco / r) * inv_D;
-        uphi = (T(2) * M * a * E_isco / r + (T(1) - T(2) * M / r) * L_isco) * inv_D;
+        const T inv_D = dv(T(1), mx(Dl, T(1e-3)));
+        const T ir = dv(T(1), r);
+        ut = ((r * r + a2 + T(2) * M * a2 * ir) * E_isco - T(2) * M * a * L_isco * ir) * inv_D;
+        uphi = (T(2) * M * a * E_isco * ir + (T(1) - T(2) * M * ir) * L_isco) * inv_D;
         const T t1 = E_isco * (r * r + a2) - a * L_isco;
         const T t2 = L_isco - a * E_isco;
         const T Rr = t1 * t1 - Dl * (r * r + t2 * t2);
-        ur = -A::psqrt(mx(Rr, T(0))) / (r * r);
+        ur = -dv(sq(mx(Rr, T(0))), r * r);
         ut += T(2) * M * r * inv_D * ur;
         uphi += a * inv_D * ur;
     }
-    const T inv_s2 = T(1) / mx(s2, T(1e-6));
+    const T inv_s2 = dv(T(1), mx(s2, T(1e-6)));
     const T cphi = (r * sx + a * sy) * inv_s2;
     const T sphi = (r * sy - a * sx) * inv_s2;
     const T ux = ur * cphi - sy * uphi;
     const T uy = ur * sphi + sx * uphi;
     const T Es = A::fabs(E) > T(1e-6) ? E : T(1e-6);
-    return clampv(ut - (psx * ux + psy * uy) / Es, T(0.1), T(20));
+    return clampv(ut - dv(psx * ux + psy * uy, Es), T(0.1), T(20));
 }
 
 }  // namespace st

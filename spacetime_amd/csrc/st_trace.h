@@ -110,20 +110,60 @@ __device__ __forceinline__ void sample_sky(const DeviceScene& D, T theta, T phi,
     col[2] = T(c00.z) * w00 + T(c10.z) * w10 + T(c01.z) * w01 + T(c11.z) * w11;
 }
 
-// Trilinear gas density, sample_volume_mtl (src/metal.jl:569-610).
+// Global-memory views of the scene's arrays.  The out-of-line gas sampler
+// reaches the scene through a generic reference; naming the address space
+// lets its gathers be global (not flat) loads.
+typedef const float __attribute__((address_space(1))) gfloat;
+
+// Constants of the gas grid and the emission model for one march call, so
+// the per-sample index arithmetic is multiplications.
 template <typename T>
-__device__ __forceinline__ T sample_volume(const DeviceScene& D, T s, T phi, T z) {
+struct GasGrid {
+    gfloat* vol;
+    gfloat* lut;     // [lut_n][4]
+    int nr, nphi, nz, lut_n;
+    T zmax, ls_in, ls_out, r_scale, p_scale, z_scale;
+    T lut_tmin, lut_tmax, lut_scale;
+    T T0, inv_Tr0;
+    int T_hold;
+    T opac, emis;
+};
+
+template <typename T>
+__device__ __forceinline__ GasGrid<T> gas_grid(const DeviceScene& D) {
+    GasGrid<T> G;
+    G.vol = (gfloat*)D.vol;
+    G.lut = (gfloat*)D.lut;
+    G.nr = D.nr; G.nphi = D.nphi; G.nz = D.nz; G.lut_n = D.lut_n;
+    G.zmax = T(D.vol_params[2]);
+    G.ls_in = T(D.vol_params[0]);
+    G.ls_out = T(D.vol_params[1]);
+    G.r_scale = T(D.nr - 1) / (G.ls_out - G.ls_in);
+    G.p_scale = T(D.nphi) / T(6.283185307179586);
+    G.z_scale = T(D.nz - 1) / (T(2) * G.zmax);
+    G.lut_tmin = T(D.lut_tmin);
+    G.lut_tmax = T(D.lut_tmax);
+    G.lut_scale = (T(D.lut_n) - T(1)) / (G.lut_tmax - G.lut_tmin);
+    G.T0 = T(D.T0);
+    G.inv_Tr0 = T(1) / T(D.Tr0);
+    G.T_hold = D.T_hold;
+    G.opac = T(D.vol_params[7]);
+    G.emis = T(D.vol_params[6]);
+    return G;
+}
+
+// Trilinear gas density, sample_volume_mtl (src/metal.jl:569-610): grid
+// coordinates (log s, phi periodic, z) and 8 taps.
+template <typename T>
+__device__ __forceinline__ T sample_volume(const GasGrid<T>& G, T s, T phi, T z) {
     using A = Ar<T>;
-    const T zmax = T(D.vol_params[2]);
-    if (s <= T(0) || A::fabs(z) >= zmax) return T(0);
+    if (s <= T(0) || A::fabs(z) >= G.zmax) return T(0);
     const T ls = GasMath<T>::log(s);
-    const T ls_in = T(D.vol_params[0]), ls_out = T(D.vol_params[1]);
-    if (ls <= ls_in || ls >= ls_out) return T(0);
-    const int nr = D.nr, nphi = D.nphi, nz = D.nz;
-    const T two_pi = T(6.283185307179586);
-    const T fr = (ls - ls_in) / (ls_out - ls_in) * T(nr - 1);
-    const T fp = jmod(phi, two_pi) / two_pi * T(nphi);
-    const T fz = (z + zmax) / (T(2) * zmax) * T(nz - 1);
+    if (ls <= G.ls_in || ls >= G.ls_out) return T(0);
+    const int nr = G.nr, nphi = G.nphi, nz = G.nz;
+    const T fr = (ls - G.ls_in) * G.r_scale;
+    const T fp = jmod(phi, T(6.283185307179586)) * G.p_scale;
+    const T fz = (z + G.zmax) * G.z_scale;
     int i0 = (int)A::floor(fr);
     i0 = i0 < 0 ? 0 : (i0 > nr - 2 ? nr - 2 : i0);
     const T tr = fr - T(i0);
@@ -136,7 +176,7 @@ __device__ __forceinline__ T sample_volume(const DeviceScene& D, T s, T phi, T z
     if (j0a < 0) j0a += nphi;
     int j1a = (j0 + 1) % nphi;
     if (j1a < 0) j1a += nphi;
-    const float* v = D.vol;
+    gfloat* v = G.vol;
     auto d = [&](int ii, int jj, int kk) { return T(v[((size_t)kk * nphi + jj) * nr + ii]); };
     const T c00 = d(i0, j0a, k0) + tr * (d(i0 + 1, j0a, k0) - d(i0, j0a, k0));
     const T c10 = d(i0, j1a, k0) + tr * (d(i0 + 1, j1a, k0) - d(i0, j1a, k0));
@@ -258,39 +298,43 @@ template <typename T>
 __device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
                                            T seg, T t0, T ds_gas, int k0, int k1, T& alpha, T acc[3], int& nsamp) {
     using A = Ar<T>;
-    const T zmax = T(D.vol_params[2]);
+    using G_ = GasMath<T>;
+    const GasGrid<T> G = gas_grid<T>(D);
+    const T inv_seg = T(1) / seg;
+    const T inv_risco = T(1) / K.r_isco;
     for (int k = k0; k <= k1 && alpha > T(0.003); ++k) {
         const T t_s = t0 + T(k) * ds_gas;
-        const T fs = t_s / seg;
+        const T fs = t_s * inv_seg;
         const T sx = sp.x + fs * (sn.x - sp.x);
         const T sy = sp.y + fs * (sn.y - sp.y);
         const T sz = sp.z + fs * (sn.z - sp.z);
-        if (A::fabs(sz) < zmax) {
-            const T s_cyl = A::psqrt(sx * sx + sy * sy);
+        if (A::fabs(sz) < G.zmax) {
+            const T s_cyl = Fast<T>::sqrt(sx * sx + sy * sy);
             if (s_cyl > T(1e-6)) {
                 ++nsamp;
-                using G = GasMath<T>;
-                T rho = sample_volume(D, s_cyl, G::atan2(sy, sx), sz);
+                T rho = sample_volume(G, s_cyl, G_::atan2(sy, sx), sz);
                 if (rho > T(1e-4)) {
-                    const T sT = D.T_hold ? mx(s_cyl, K.r_isco) : s_cyl;
-                    const T T_emit = T(D.T0) * G::exp(T(-0.75) * G::log(mx(sT / T(D.Tr0), T(1e-6))));
+                    const T sT = G.T_hold ? mx(s_cyl, K.r_isco) : s_cyl;
+                    const T T_emit = G.T0 * G_::exp(T(-0.75) * G_::log(mx(sT * G.inv_Tr0, T(1e-6))));
                     if (s_cyl < K.r_isco) {
-                        const T q = s_cyl / K.r_isco;
+                        const T q = s_cyl * inv_risco;
                         rho *= q * q * q;
                     }
                     const T psx = sp.px + fs * (sn.px - sp.px);
                     const T psy = sp.py + fs * (sn.py - sp.py);
-                    const T opz = emitter_opz(sx, sy, psx, psy, -K.pt, K.M, K.a, K.r_isco, K.E_isco, K.L_isco);
-                    const T T_obs = T_emit * K.scam / opz;
-                    const T inten = T(100) / (G::exp(T(29622.4) / mx(T_obs, T(1))) - T(1));
-                    T col[3];
-                    lut_color(D, T_obs, col);
-                    const T tau = T(D.vol_params[7]) * rho * ds_gas;
-                    const T aa = T(1) - G::exp(-tau);
-                    const T w = alpha * aa * inten * T(D.vol_params[6]);
-                    acc[0] += w * col[0];
-                    acc[1] += w * col[1];
-                    acc[2] += w * col[2];
+                    const T opz = emitter_opz<T, true>(sx, sy, psx, psy, -K.pt, K.M, K.a, K.r_isco, K.E_isco,
+                                                       K.L_isco);
+                    const T T_obs = Fast<T>::div(T_emit * K.scam, opz);
+                    const T inten = Fast<T>::div(T(100), G_::exp(Fast<T>::div(T(29622.4), mx(T_obs, T(1)))) - T(1));
+                    int li = (int)((clampv(T_obs, G.lut_tmin, G.lut_tmax) - G.lut_tmin) * G.lut_scale + T(0.5));
+                    li = li < 0 ? 0 : (li > G.lut_n - 1 ? G.lut_n - 1 : li);
+                    gfloat* c = G.lut + 4 * li;
+                    const T tau = G.opac * rho * ds_gas;
+                    const T aa = T(1) - G_::exp(-tau);
+                    const T w = alpha * aa * inten * G.emis;
+                    acc[0] += w * T(c[0]);
+                    acc[1] += w * T(c[1]);
+                    acc[2] += w * T(c[2]);
                     alpha *= (T(1) - aa);
                 }
             }
