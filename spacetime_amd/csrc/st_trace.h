@@ -637,19 +637,26 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
         const T r2 = s.x * s.x + s.y * s.y + s.z * s.z;
         const T r = (sizeof(T) == 4) ? r2 * A::rsq(r2) : A::sqrt(r2);
         if (P.rec) rk_min = mn(rk_min, rk2);
+        // the step's exit tests, evaluated together and taken in one branch
+        // (capture first, then promotion, then escape)
+        bool cap = false;
         if (P.capture_mode == 1) {
-            if ((rk2 < rcap2 && rk2 < rk2_prev) || rk2 < rfloor2) { hit = true; break; }
+            cap = (rk2 < rcap2 && rk2 < rk2_prev) || rk2 < rfloor2;
             rk2_prev = rk2;
         } else if (KERR) {
-            if (rk2 < rkill2) { hit = true; break; }
+            cap = rk2 < rkill2;
         } else if (r < T(3.2) * M) {
-            if (r < r_floor || (r < T(2.95) * M && r_prev > T(0) && r < r_prev - T(1e-4) * M)) { hit = true; break; }
+            cap = r < r_floor || (r < T(2.95) * M && r_prev > T(0) && r < r_prev - T(1e-4) * M);
             r_prev = r;
         }
-        if constexpr (ROUTE) {
-            if (rk2 < rprom2) { routed = true; break; }
+        const bool rt = ROUTE && rk2 < rprom2;
+        const bool es = r > r_escape;
+        if (cap || rt || es) {
+            hit = cap;
+            routed = !cap && rt;
+            escaped = !cap && !rt && es;
+            break;
         }
-        if (r > r_escape) { escaped = true; break; }
         const T hcap = (VOL && r2 < vol_rb2) ? T(2) : h_cap;
         T h = dt * mn(mx(h_coef * r * invM, h_floor), hcap);
         if (KERR && P.kerr_shrink) {
