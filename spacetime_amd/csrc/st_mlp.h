@@ -72,8 +72,8 @@ hipError_t launch_make_samples(const DeviceScene* D, const TraceParams& P, const
                                int per_cam, int n, size_t total, float* X, float* T, int nblocks, hipStream_t s);
 hipError_t launch_fragment_weights(const float* P, bf16* W2f, bf16* W3f, bf16* W4f, bf16* W5f, hipStream_t s);
 hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const bf16* W4f, const bf16* W5f,
-                        const InferView& V, float* pred, int npix, hipStream_t s);
-hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene* D, float* out, float* rec,
+                        const InferView& V, float* pred, unsigned char* esc, int npix, hipStream_t s);
+hipError_t launch_select(const float* pred, const unsigned char* esc, const InferView& V, const DeviceScene* D, float* out, float* rec,
                          float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s);
 hipError_t launch_gather(const float* X, const float* T, size_t n, uint64_t seed, float* Xb, float* Tb, int B,
                          hipStream_t s);

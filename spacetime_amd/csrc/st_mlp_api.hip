@@ -73,10 +73,16 @@ int prepare_infer(st_mlp* m, st_ctx* c, size_t npix) {
     if (m->pred_cap < npix) {
         dfree(m->pred);
         m->pred = nullptr;
-        ST_HIP(hipMalloc(&m->pred, npix * 8 * sizeof(float)));
+        // predictions (8 floats per pixel) followed by one escape byte per pixel
+        ST_HIP(hipMalloc(&m->pred, npix * (8 * sizeof(float) + 1)));
         m->pred_cap = npix;
     }
     return ST_OK;
+}
+
+// The per-pixel escape bytes behind the npix x 8 prediction floats.
+unsigned char* esc_bytes(st_mlp* m, int npix) {
+    return reinterpret_cast<unsigned char*>(m->pred + (size_t)npix * 8);
 }
 
 int make_view(const st_surrogate_view* v, float a, int width, int height, float ju, float jv, InferView& V,
@@ -423,7 +429,7 @@ int st_mlp_infer(st_mlp* m, st_ctx* c, const st_surrogate_view* v, float spin, f
     int rc = make_view(v, spin, c->width, c->height, 0.5f, 0.5f, V, cam);
     if (rc != ST_OK) return rc;
     if ((rc = prepare_infer(m, c, (size_t)npix)) != ST_OK) return rc;
-    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, npix, c->stream));
+    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, esc_bytes(m, npix), npix, c->stream));
     ST_HIP(hipMemcpyAsync(pred_host, m->pred, (size_t)npix * 8 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     ST_HIP(hipStreamSynchronize(c->stream));
     return ST_OK;
@@ -457,8 +463,8 @@ int st_mlp_render(st_mlp* m, st_ctx* c, const st_surrogate_view* v, const float 
     const bool kerr = sp[6] != 0.0f;
     ST_HIP(hipEventRecord(c->ev[0], c->stream));
     ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
-    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, npix, c->stream));
-    ST_HIP(launch_select(m->pred, V, D, c->out, F.rec, a->weight, c->queue, c->qcount, npix, c->stream));
+    ST_HIP(launch_infer(m->P, m->W2f, m->W3f, m->W4f, m->W5f, V, m->pred, esc_bytes(m, npix), npix, c->stream));
+    ST_HIP(launch_select(m->pred, esc_bytes(m, npix), V, D, c->out, F.rec, a->weight, c->queue, c->qcount, npix, c->stream));
     ST_HIP(hipEventRecord(c->ev[1], c->stream));
     ST_HIP(st::launch_queue32(kerr, false, D, P, F, (size_t)npix, c->stream));
     ST_HIP(hipEventRecord(c->ev[2], c->stream));

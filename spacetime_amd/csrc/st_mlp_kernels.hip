@@ -385,11 +385,12 @@ typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 // Layer 1 has 3 of its 5 inputs constant over the frame (r, theta, a): its
 // constant part is folded per unit, leaving 2 FMAs per (unit, ray).  Wave w
 // computes units [64w, 64w + 64) of every hidden layer for all 128 rays.
-// pred[p] = (d_world xyz, p_escape, |correction|, unused).
+// pred[p] = (d_world xyz, p_escape, |correction|, unused); esc[p] = p_escape >= 1/2.
 __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ P, const bf16* __restrict__ W2f,
                                                       const bf16* __restrict__ W3f, const bf16* __restrict__ W4f,
                                                       const bf16* __restrict__ W5f, InferView V,
-                                                      float* __restrict__ pred, int npix) {
+                                                      float* __restrict__ pred, unsigned char* __restrict__ esc,
+                                                      int npix) {
     constexpr int ROWS = 128, LD = HID + 8;
     __shared__ __attribute__((aligned(16))) bf16 Hs[ROWS * LD];
     __shared__ float U[3][HID];       // layer 1: frame-constant part, alpha weight, beta weight
@@ -520,6 +521,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
         o[2] = cx * V.axes[2] + cy * V.axes[5] + cz * V.axes[8];
         o[3] = e0 / (e0 + e1);
         o[4] = sqrtf(dx * dx + dy * dy + dz * dz);
+        esc[base + tid] = o[3] >= 0.5f ? 1 : 0;
     }
 }
 
@@ -528,7 +530,8 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(const float* __restrict__ 
 // predicted fate (shadow edge), or when the predicted correction exceeds
 // delta_max (strong lensing near the photon ring).  Fallback pixels are
 // appended to the queue (one atomic per wave); the rest are shaded here.
-__global__ __launch_bounds__(256) void k_surrogate_select(const float* __restrict__ pred, InferView V, const DeviceScene* __restrict__ Dp,
+__global__ __launch_bounds__(256) void k_surrogate_select(const float* __restrict__ pred,
+                                                          const unsigned char* __restrict__ esc_bits, InferView V, const DeviceScene* __restrict__ Dp,
                                                           float* __restrict__ out, float* __restrict__ rec,
                                                           float weight, uint2* __restrict__ queue,
                                                           unsigned* __restrict__ qcount, int npix) {
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(256) void k_surrogate_select(const float* __restric
             for (int di = -V.edge_px; di <= V.edge_px; ++di) {
                 const int ii = i + di;
                 if (ii < 0 || ii >= V.width) continue;
-                if ((pred[((size_t)jj * V.width + ii) * 8 + 3] >= 0.5f) != esc) { push = true; break; }
+                if ((esc_bits[(size_t)jj * V.width + ii] != 0) != esc) { push = true; break; }
             }
         }
         if (!push) {
@@ -649,14 +652,15 @@ hipError_t launch_fragment_weights(const float* P, bf16* W2f, bf16* W3f, bf16* W
 }
 
 hipError_t launch_infer(const float* P, const bf16* W2f, const bf16* W3f, const bf16* W4f, const bf16* W5f,
-                        const InferView& V, float* pred, int npix, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3((npix + 127) / 128), dim3(256), 0, s, P, W2f, W3f, W4f, W5f, V, pred, npix);
+                        const InferView& V, float* pred, unsigned char* esc, int npix, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3((npix + 127) / 128), dim3(256), 0, s, P, W2f, W3f, W4f, W5f, V, pred, esc,
+                       npix);
     return hipGetLastError();
 }
 
-hipError_t launch_select(const float* pred, const InferView& V, const DeviceScene* D, float* out, float* rec,
+hipError_t launch_select(const float* pred, const unsigned char* esc, const InferView& V, const DeviceScene* D, float* out, float* rec,
                          float weight, uint2* queue, unsigned* qcount, int npix, hipStream_t s) {
-    hipLaunchKernelGGL(k_surrogate_select, dim3((npix + 255) / 256), dim3(256), 0, s, pred, V, D, out, rec, weight,
+    hipLaunchKernelGGL(k_surrogate_select, dim3((npix + 255) / 256), dim3(256), 0, s, pred, esc, V, D, out, rec, weight,
                        queue, qcount, npix);
     return hipGetLastError();
 }
