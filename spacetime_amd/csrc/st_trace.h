@@ -258,6 +258,15 @@ struct Kin {  // disc kinematics shared by the shading blocks
     T r_isco, E_isco, L_isco, scam, pt, M, a;
 };
 
+// Radiance, transmittance and sample count handed to the out-of-line gas
+// sampler as one block.
+template <typename T>
+struct ShadeState {
+    T acc[3];
+    T alpha;
+    int nsamp;
+};
+
 // Thin-plane crossing shading, src/metal.jl:1524-1569.
 template <typename T>
 __device__ __noinline__ void shade_disc(const DeviceScene& D, const Kin<T>& K, T xh, T yh, T pxh, T pyh,
@@ -296,12 +305,18 @@ __device__ __noinline__ void shade_disc(const DeviceScene& D, const Kin<T>& K, T
 // Arc-length gas march over one accepted step, src/metal.jl:1440-1513.
 template <typename T>
 __device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K, const S6<T>& sp, const S6<T>& sn,
-                                           T seg, T t0, T ds_gas, int k0, int k1, T& alpha, T acc[3], int& nsamp) {
+                                           T seg, T t0, T ds_gas, int k0, int k1, ShadeState<T>& st) {
     using A = Ar<T>;
     using G_ = GasMath<T>;
     const GasGrid<T> G = gas_grid<T>(D);
     const T inv_seg = T(1) / seg;
     const T inv_risco = T(1) / K.r_isco;
+    // updated in place: register copies of the block raise this function's
+    // VGPR count from 104 to 151, which would cap every volumetric kernel at
+    // 3 waves per SIMD instead of 4
+    T& alpha = st.alpha;
+    T* acc = st.acc;
+    int& nsamp = st.nsamp;
     for (int k = k0; k <= k1 && alpha > T(0.003); ++k) {
         const T t_s = t0 + T(k) * ds_gas;
         const T fs = t_s * inv_seg;
@@ -375,7 +390,13 @@ __device__ __forceinline__ void march_gas(const DeviceScene& D, const Kin<T>& K,
     } else if (!(A::fabs(sp.z) < zmax)) {
         k1 = -1;
     }
-    if (k0 <= k1) march_samples(D, K, sp, sn, seg, t0, ds_gas, k0, k1, alpha, acc, nsamp);
+    if (k0 <= k1) {
+        ShadeState<T> st = {{acc[0], acc[1], acc[2]}, alpha, nsamp};
+        march_samples(D, K, sp, sn, seg, t0, ds_gas, k0, k1, st);
+        acc[0] = st.acc[0]; acc[1] = st.acc[1]; acc[2] = st.acc[2];
+        alpha = st.alpha;
+        nsamp = st.nsamp;
+    }
     march_acc = seg - (t0 + T(n - 1) * ds_gas);
 }
 
@@ -609,8 +630,15 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     const T invM = T(1) / M;
     const T r_floor = r_cam > T(2.05) * M ? T(2) * M : T(0.3) * M;
     T r_prev = T(-1);
-    T alpha = T(1);
+    // Transmittance and accumulated radiance.  The out-of-line shaders take
+    // them by reference, so they live on the stack; volumetric kernels, whose
+    // step loop also tests the transmittance before each gas march, keep a
+    // register copy synchronised around the calls (for the thin disc alone
+    // the stack copy measured faster).
     T acc[3] = {T(0), T(0), T(0)};
+    T alpha_mem = T(1), alpha_reg = T(1);
+    T& alpha = VOL ? alpha_reg : alpha_mem;
+    int nsamp = 0;
     T march_acc = T(0);
     T rk_min = T(1e30);
     // previous step start radius and velocity (escape-direction interpolation)
@@ -776,7 +804,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             const T rn2 = s.x * s.x + s.y * s.y + s.z * s.z;
             if (alpha > T(0.003) && mn(r2, rn2) < vol_rb2) {
                 const T vlen = mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
-                march_gas(D, K, sp, s, h_used * vlen, ds_gas, march_acc, alpha, acc, o.gas_samples);
+                march_gas(D, K, sp, s, h_used * vlen, ds_gas, march_acc, alpha, acc, nsamp);
                 if (alpha < T(0.003)) break;
             }
         }
@@ -786,11 +814,16 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             // only crossings near the annulus reach the out-of-line shader
             // (which applies the exact inner < s < outer test itself)
             const T s2 = xh * xh + yh * yh;
-            if (s2 > disc_in2 && s2 < disc_out2)
-                shade_disc(D, K, xh, yh, sp.px + cfr * (s.px - sp.px), sp.py + cfr * (s.py - sp.py), alpha, acc);
+            if (s2 > disc_in2 && s2 < disc_out2) {
+                if constexpr (VOL) alpha_mem = alpha;
+                shade_disc(D, K, xh, yh, sp.px + cfr * (s.px - sp.px), sp.py + cfr * (s.py - sp.py), alpha_mem,
+                           acc);
+                if constexpr (VOL) alpha = alpha_mem;
+            }
         }
     }
     o.steps = steps;
+    o.gas_samples = nsamp;
     if constexpr (ROUTE) {
         if (routed) {
             o.routed = true;
