@@ -256,6 +256,8 @@ int st_ctx_set_volume(st_ctx* c, const float* vol, int nr, int nphi, int nz, con
     if (nr < 2 || nphi < 1 || nz < 2) return fail(ST_ERR_INVALID_ARG, "volume must be at least 2x1x2");
     if (!(vp[1] > vp[0]) || !(vp[2] > 0.0f)) return fail(ST_ERR_INVALID_ARG, "bad vol_params");
     const size_t n = (size_t)nr * nphi * nz;
+    // the gas sampler addresses the grid with 32-bit byte offsets
+    if (n >= ((size_t)1 << 30)) return fail(ST_ERR_INVALID_ARG, "volume grid must hold fewer than 2^30 cells");
     float* d = nullptr;
     ST_HIP(hipMalloc(&d, sizeof(float) * n));
     ST_HIP(hipMemcpy(d, vol, sizeof(float) * n, hipMemcpyHostToDevice));

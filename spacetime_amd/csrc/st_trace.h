@@ -129,59 +129,93 @@ struct GasGrid {
     T opac, emis;
 };
 
+// Values that are the same on every lane (the scene block is one per
+// launch).  An out-of-line function receives the block's address in vector
+// registers, so whatever it derives from the block would take one VGPR per
+// value; readfirstlane moves each to scalar registers.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uni(double x) {
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ gfloat* uni(gfloat* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (gfloat*)(((unsigned long long)hi << 32) | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ GasGrid<T> gas_grid(const DeviceScene& D) {
     GasGrid<T> G;
-    G.vol = (gfloat*)D.vol;
-    G.lut = (gfloat*)D.lut;
-    G.nr = D.nr; G.nphi = D.nphi; G.nz = D.nz; G.lut_n = D.lut_n;
-    G.zmax = T(D.vol_params[2]);
-    G.ls_in = T(D.vol_params[0]);
-    G.ls_out = T(D.vol_params[1]);
-    G.r_scale = T(D.nr - 1) / (G.ls_out - G.ls_in);
-    G.p_scale = T(D.nphi) / T(6.283185307179586);
-    G.z_scale = T(D.nz - 1) / (T(2) * G.zmax);
-    G.lut_tmin = T(D.lut_tmin);
-    G.lut_tmax = T(D.lut_tmax);
-    G.lut_scale = (T(D.lut_n) - T(1)) / (G.lut_tmax - G.lut_tmin);
-    G.T0 = T(D.T0);
-    G.inv_Tr0 = T(1) / T(D.Tr0);
-    G.T_hold = D.T_hold;
-    G.opac = T(D.vol_params[7]);
-    G.emis = T(D.vol_params[6]);
+    G.vol = uni((gfloat*)D.vol);
+    G.lut = uni((gfloat*)D.lut);
+    G.nr = uni(D.nr); G.nphi = uni(D.nphi); G.nz = uni(D.nz); G.lut_n = uni(D.lut_n);
+    G.zmax = uni(T(D.vol_params[2]));
+    G.ls_in = uni(T(D.vol_params[0]));
+    G.ls_out = uni(T(D.vol_params[1]));
+    G.r_scale = uni(T(D.nr - 1) / (G.ls_out - G.ls_in));
+    G.p_scale = uni(T(D.nphi) / T(6.283185307179586));
+    G.z_scale = uni(T(D.nz - 1) / (T(2) * G.zmax));
+    G.lut_tmin = uni(T(D.lut_tmin));
+    G.lut_tmax = uni(T(D.lut_tmax));
+    G.lut_scale = uni((T(D.lut_n) - T(1)) / (G.lut_tmax - G.lut_tmin));
+    G.T0 = uni(T(D.T0));
+    G.inv_Tr0 = uni(T(1) / T(D.Tr0));
+    G.T_hold = uni(D.T_hold);
+    G.opac = uni(T(D.vol_params[7]));
+    G.emis = uni(T(D.vol_params[6]));
     return G;
 }
 
 // Trilinear gas density, sample_volume_mtl (src/metal.jl:569-610): grid
-// coordinates (log s, phi periodic, z) and 8 taps.
+// coordinates (log s, phi periodic, z) and 8 taps.  Written without branches:
+// every index is clamped into the grid, so the taps are always in bounds and
+// can be issued before it is known whether the point lies inside the grid.
+// Returns the interpolated value and whether the point is inside; outside,
+// the density is 0.  Inside the grid the clamps change nothing.  phi is an
+// atan2 result (|phi| <= pi), so one conditional add wraps it.
 template <typename T>
-__device__ __forceinline__ T sample_volume(const GasGrid<T>& G, T s, T phi, T z) {
+__device__ __forceinline__ T sample_volume(const GasGrid<T>& G, T s, T phi, T z, bool& inside) {
     using A = Ar<T>;
-    if (s <= T(0) || A::fabs(z) >= G.zmax) return T(0);
     const T ls = GasMath<T>::log(s);
-    if (ls <= G.ls_in || ls >= G.ls_out) return T(0);
+    inside = s > T(0) && A::fabs(z) < G.zmax && ls > G.ls_in && ls < G.ls_out;
     const int nr = G.nr, nphi = G.nphi, nz = G.nz;
-    const T fr = (ls - G.ls_in) * G.r_scale;
-    const T fp = jmod(phi, T(6.283185307179586)) * G.p_scale;
-    const T fz = (z + G.zmax) * G.z_scale;
+    const T fr = clampv((ls - G.ls_in) * G.r_scale, T(0), T(nr - 1));
+    T wp = phi < T(0) ? phi + T(6.283185307179586) : phi;
+    wp = wp >= T(6.283185307179586) ? wp - T(6.283185307179586) : wp;
+    const T fp = clampv(wp * G.p_scale, T(0), T(nphi));
+    const T fz = clampv((z + G.zmax) * G.z_scale, T(0), T(nz - 1));
     int i0 = (int)A::floor(fr);
-    i0 = i0 < 0 ? 0 : (i0 > nr - 2 ? nr - 2 : i0);
+    i0 = i0 > nr - 2 ? nr - 2 : i0;
     const T tr = fr - T(i0);
     const int j0 = (int)A::floor(fp);
     const T tp = fp - T(j0);
     int k0 = (int)A::floor(fz);
-    k0 = k0 < 0 ? 0 : (k0 > nz - 2 ? nz - 2 : k0);
+    k0 = k0 > nz - 2 ? nz - 2 : k0;
     const T tz = fz - T(k0);
-    int j0a = j0 % nphi;
-    if (j0a < 0) j0a += nphi;
-    int j1a = (j0 + 1) % nphi;
-    if (j1a < 0) j1a += nphi;
-    gfloat* v = G.vol;
-    auto d = [&](int ii, int jj, int kk) { return T(v[((size_t)kk * nphi + jj) * nr + ii]); };
-    const T c00 = d(i0, j0a, k0) + tr * (d(i0 + 1, j0a, k0) - d(i0, j0a, k0));
-    const T c10 = d(i0, j1a, k0) + tr * (d(i0 + 1, j1a, k0) - d(i0, j1a, k0));
-    const T c01 = d(i0, j0a, k0 + 1) + tr * (d(i0 + 1, j0a, k0 + 1) - d(i0, j0a, k0 + 1));
-    const T c11 = d(i0, j1a, k0 + 1) + tr * (d(i0 + 1, j1a, k0 + 1) - d(i0, j1a, k0 + 1));
+    const int j0a = j0 >= nphi ? j0 - nphi : j0;
+    const int j1a = j0 + 1 >= nphi ? j0 + 1 - nphi : j0 + 1;
+    // 32-bit byte offsets from the scalar base address (the grid holds
+    // fewer than 2^30 cells), so each tap is one load with no 64-bit
+    // address arithmetic
+    typedef const char __attribute__((address_space(1))) gchar;
+    gchar* base = (gchar*)G.vol;
+    // radial neighbours (i0, i0 + 1) sit 4 bytes apart: one offset per pair
+    auto lerp_r = [&](int jj, int kk) {
+        const unsigned off = (((unsigned)kk * (unsigned)nphi + (unsigned)jj) * (unsigned)nr + (unsigned)i0) << 2;
+        const T a0 = T(*(gfloat*)(base + off)), a1 = T(*(gfloat*)(base + off + 4u));
+        return a0 + tr * (a1 - a0);
+    };
+    const T c00 = lerp_r(j0a, k0);
+    const T c10 = lerp_r(j1a, k0);
+    const T c01 = lerp_r(j0a, k0 + 1);
+    const T c11 = lerp_r(j1a, k0 + 1);
     const T c0 = c00 + tp * (c10 - c00), c1 = c01 + tp * (c11 - c01);
     return c0 + tz * (c1 - c0);
 }
@@ -311,50 +345,69 @@ __device__ __noinline__ void march_samples(const DeviceScene& D, const Kin<T>& K
     const GasGrid<T> G = gas_grid<T>(D);
     const T inv_seg = T(1) / seg;
     const T inv_risco = T(1) / K.r_isco;
-    // updated in place: register copies of the block raise this function's
-    // VGPR count from 104 to 151, which would cap every volumetric kernel at
-    // 3 waves per SIMD instead of 4
-    T& alpha = st.alpha;
-    T* acc = st.acc;
-    int& nsamp = st.nsamp;
-    for (int k = k0; k <= k1 && alpha > T(0.003); ++k) {
+    const T r_isco = K.r_isco, E_isco = K.E_isco, L_isco = K.L_isco, scam = K.scam, ptn = -K.pt, M = K.M, a = K.a;
+    // segment end points in registers (through the references they would be
+    // reloaded after every store to the shading block)
+    const T x0 = sp.x, y0 = sp.y, z0 = sp.z, px0 = sp.px, py0 = sp.py;
+    const T dx = sn.x - sp.x, dy = sn.y - sp.y, dz = sn.z - sp.z, dpx = sn.px - sp.px, dpy = sn.py - sp.py;
+    T alpha = st.alpha;
+    T acc0 = st.acc[0], acc1 = st.acc[1], acc2 = st.acc[2];
+    int nsamp = st.nsamp;
+    struct Pt {
+        T fs, sx, sy, sz, s_cyl;
+        bool in;
+    };
+    auto point = [&](int k) {
+        Pt q;
         const T t_s = t0 + T(k) * ds_gas;
-        const T fs = t_s * inv_seg;
-        const T sx = sp.x + fs * (sn.x - sp.x);
-        const T sy = sp.y + fs * (sn.y - sp.y);
-        const T sz = sp.z + fs * (sn.z - sp.z);
-        if (A::fabs(sz) < G.zmax) {
-            const T s_cyl = Fast<T>::sqrt(sx * sx + sy * sy);
-            if (s_cyl > T(1e-6)) {
-                ++nsamp;
-                T rho = sample_volume(G, s_cyl, G_::atan2(sy, sx), sz);
-                if (rho > T(1e-4)) {
-                    const T sT = G.T_hold ? mx(s_cyl, K.r_isco) : s_cyl;
-                    const T T_emit = G.T0 * G_::exp(T(-0.75) * G_::log(mx(sT * G.inv_Tr0, T(1e-6))));
-                    if (s_cyl < K.r_isco) {
-                        const T q = s_cyl * inv_risco;
-                        rho *= q * q * q;
-                    }
-                    const T psx = sp.px + fs * (sn.px - sp.px);
-                    const T psy = sp.py + fs * (sn.py - sp.py);
-                    const T opz = emitter_opz<T, true>(sx, sy, psx, psy, -K.pt, K.M, K.a, K.r_isco, K.E_isco,
-                                                       K.L_isco);
-                    const T T_obs = Fast<T>::div(T_emit * K.scam, opz);
-                    const T inten = Fast<T>::div(T(100), G_::exp(Fast<T>::div(T(29622.4), mx(T_obs, T(1)))) - T(1));
-                    int li = (int)((clampv(T_obs, G.lut_tmin, G.lut_tmax) - G.lut_tmin) * G.lut_scale + T(0.5));
-                    li = li < 0 ? 0 : (li > G.lut_n - 1 ? G.lut_n - 1 : li);
-                    gfloat* c = G.lut + 4 * li;
-                    const T tau = G.opac * rho * ds_gas;
-                    const T aa = T(1) - G_::exp(-tau);
-                    const T w = alpha * aa * inten * G.emis;
-                    acc[0] += w * T(c[0]);
-                    acc[1] += w * T(c[1]);
-                    acc[2] += w * T(c[2]);
-                    alpha *= (T(1) - aa);
-                }
-            }
+        q.fs = t_s * inv_seg;
+        q.sx = x0 + q.fs * dx;
+        q.sy = y0 + q.fs * dy;
+        q.sz = z0 + q.fs * dz;
+        q.s_cyl = Fast<T>::sqrt(q.sx * q.sx + q.sy * q.sy);
+        q.in = A::fabs(q.sz) < G.zmax && q.s_cyl > T(1e-6);
+        return q;
+    };
+    auto shade = [&](const Pt& q, T rho) {
+        if (!q.in) return;
+        ++nsamp;
+        if (!(rho > T(1e-4))) return;
+        const T sT = G.T_hold ? mx(q.s_cyl, r_isco) : q.s_cyl;
+        const T T_emit = G.T0 * G_::exp(T(-0.75) * G_::log(mx(sT * G.inv_Tr0, T(1e-6))));
+        if (q.s_cyl < r_isco) {
+            const T qq = q.s_cyl * inv_risco;
+            rho *= qq * qq * qq;
         }
+        const T psx = px0 + q.fs * dpx;
+        const T psy = py0 + q.fs * dpy;
+        const T opz = emitter_opz<T, true>(q.sx, q.sy, psx, psy, ptn, M, a, r_isco, E_isco, L_isco);
+        const T T_obs = Fast<T>::div(T_emit * scam, opz);
+        const T inten = Fast<T>::div(T(100), G_::exp(Fast<T>::div(T(29622.4), mx(T_obs, T(1)))) - T(1));
+        int li = (int)((clampv(T_obs, G.lut_tmin, G.lut_tmax) - G.lut_tmin) * G.lut_scale + T(0.5));
+        li = li < 0 ? 0 : (li > G.lut_n - 1 ? G.lut_n - 1 : li);
+        gfloat* c = G.lut + 4 * li;
+        const T cr = T(c[0]), cg = T(c[1]), cb = T(c[2]);
+        const T tau = G.opac * rho * ds_gas;
+        const T aa = T(1) - G_::exp(-tau);
+        const T w = alpha * aa * inten * G.emis;
+        acc0 += w * cr;
+        acc1 += w * cg;
+        acc2 += w * cb;
+        alpha *= (T(1) - aa);
+    };
+    for (int k = k0; k <= k1 && alpha > T(0.003); k += 2) {
+        const bool two = k + 1 <= k1;
+        const Pt pa = point(k), pb = point(two ? k + 1 : k);
+        bool ia, ib;
+        T va = sample_volume(G, pa.s_cyl, G_::atan2(pa.sy, pa.sx), pa.sz, ia);
+        T vb = sample_volume(G, pb.s_cyl, G_::atan2(pb.sy, pb.sx), pb.sz, ib);
+        asm volatile("" : "+v"(va), "+v"(vb));
+        shade(pa, ia ? va : T(0));
+        if (two && alpha > T(0.003)) shade(pb, ib ? vb : T(0));
     }
+    st.acc[0] = acc0; st.acc[1] = acc1; st.acc[2] = acc2;
+    st.alpha = alpha;
+    st.nsamp = nsamp;
 }
 
 // Arc-length gas march over one accepted step (src/metal.jl:1440-1513).
