@@ -162,6 +162,37 @@ int st_download_records(st_ctx* ctx, float* host_rec);
 /* Counters and device timings of the last trace call (synchronises). */
 int st_get_stats(st_ctx* ctx, st_stats* out);
 
+/* ---- baked warp maps (src/metal.jl:3173-3372) ----
+ * A warp map is mapw x maph texels of 8 floats (escape direction x, y, z —
+ * zero unless escaped; premultiplied disc/gas emission r, g, b; transmittance
+ * to the sky; escape flag), float[maph][mapw][8], the (8, mapw, maph) array of
+ * bake_warp_map.  Maps live in device memory on the context's device. */
+typedef struct st_map st_map;
+#define ST_MAP_CHANNELS 8
+int st_map_create(st_ctx* ctx, int mapw, int maph, st_map** out);
+int st_map_destroy(st_map* map);
+/* Trace one map: the tracer's BAKE output (src/metal.jl:1577-1611) of a
+ * _launch_trace! over the whole map (bake_warp_map, src/metal.jl:3197-3230;
+ * the caller packs an equirect camera, projection 2).  args->width/height
+ * must equal the map's size; the whole map is traced with args' step rule,
+ * jitter and order; precision FP32 or FP64.  Asynchronous on the context's
+ * stream. */
+int st_bake_warp_map(st_ctx* ctx, st_map* map, const float cam_params[49], const double* cam_params64,
+                     const float spacetime_params[8], const st_trace_args* args);
+/* Copy a map to host float[maph][mapw][8] (nfloats must be 8*mapw*maph). */
+int st_map_download(st_ctx* ctx, const st_map* map, float* host, size_t nfloats);
+/* Copy host float[maph][mapw][8] into a map. */
+int st_map_upload(st_ctx* ctx, st_map* map, const float* host, size_t nfloats);
+/* Composite one frame from two maps (warp_sample_kernel!,
+ * src/metal.jl:3312-3372) into the context's (3, W, H) buffer, assigned:
+ * no geodesic is integrated.  params[24] is the kernel's parameter block
+ * (0-based): 0 fov factor, 1-3 camera forward, 4-6 right, 7-9 up (world),
+ * 10-11 cos/sin of map0's bake azimuth, 12-13 map1's, 14-15 cos/sin of the
+ * current azimuth, 19 blend weight (0: map0, 1: map1), 20 exposure; the
+ * context's sky and starfield are sampled against the blended direction.
+ * Asynchronous on the context's stream. */
+int st_warp_sample(st_ctx* ctx, const st_map* map0, const st_map* map1, const float params[24]);
+
 /* ---- batched tracing: many cameras / frames of uneven size in one launch ----
  * No counterpart in the reference (it loops _launch_trace! per frame,
  * src/metal.jl:2131-2189).  Frame k is traced with cam_params[k*49 ...] into

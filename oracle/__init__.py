@@ -75,6 +75,10 @@ _SIG = {
     "ora_render": (C.c_longlong, [C.POINTER(OraScene), C.c_int, C.c_double, C.c_int, C.c_int, _F, C.c_double,
                                   C.c_int]),
     "ora_trace_list": (None, [C.POINTER(OraScene), C.c_int, C.c_double, C.c_int, _I, _D, C.c_int]),
+    "ora_bake_list": (None, [C.POINTER(OraScene), C.c_int, _I, _D, C.c_int]),
+    "ora_warp_lookup": (None, [_F, C.c_int, C.c_int, _D, C.c_double, C.c_double, _D]),
+    "ora_warp_sample": (None, [C.POINTER(OraScene), _F, C.c_int, C.c_int, _F, C.c_int, C.c_int, _D, C.c_int,
+                               C.c_int, _D]),
     "ora_tsit5_tableau": (None, [_D, _D, _D, _D]),
 }
 
@@ -264,12 +268,41 @@ class Scene:
                              rec.ctypes.data_as(_D), nthreads)
         return rec
 
+    def bake_list(self, ij: np.ndarray, nthreads=0) -> np.ndarray:
+        """Warp-map texels (n, 8) of the listed pixels (the tracer's bake output)."""
+        ij = np.ascontiguousarray(ij, dtype=np.int32)
+        tex = np.zeros((ij.shape[0], 8))
+        lib().ora_bake_list(self.ptr, ij.shape[0], ij.ctypes.data_as(_I), tex.ctypes.data_as(_D), nthreads)
+        return tex
+
+    def warp_sample(self, m0: np.ndarray, m1: np.ndarray, p, width: int, height: int) -> np.ndarray:
+        """warp_sample_kernel! over a width x height frame from two maps
+        float32 (maph, mapw, 8), with this scene's sky and stars: (H, W, 3)."""
+        m0 = np.ascontiguousarray(m0, dtype=np.float32)
+        m1 = np.ascontiguousarray(m1, dtype=np.float32)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.zeros((height, width, 3))
+        lib().ora_warp_sample(self.ptr, m0.ctypes.data_as(_F), m0.shape[1], m0.shape[0], m1.ctypes.data_as(_F),
+                              m1.shape[1], m1.shape[0], pp.ctypes.data_as(_D), width, height,
+                              out.ctypes.data_as(_D))
+        return out
+
     def render(self, method="rk4", tol=1e-6, row0=0, rows=None, nthreads=0, weight=1.0):
         rows = self.s.height - row0 if rows is None else rows
         out = np.zeros((self.s.height, self.s.width, 3), dtype=np.float32)
         steps = lib().ora_render(self.ptr, 0 if method == "rk4" else 1, tol, row0, rows, out.ctypes.data_as(_F),
                                  weight, nthreads)
         return out, steps
+
+
+def warp_lookup(m: np.ndarray, d, cphi: float, sphi: float) -> np.ndarray:
+    """_warp_lookup: bilinear 8-channel sample of a canonical map at world direction d."""
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    dd = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.zeros(8)
+    lib().ora_warp_lookup(m.ctypes.data_as(_F), m.shape[1], m.shape[0], dd.ctypes.data_as(_D), float(cphi),
+                          float(sphi), out.ctypes.data_as(_D))
+    return out
 
 
 REC_FIELDS = ["fate", "steps", "dx", "dy", "dz", "r", "g", "b", "ham", "r_end", "rk_min", "ncross",

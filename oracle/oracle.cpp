@@ -1243,4 +1243,84 @@ void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const in
     }
 }
 
+void ora_bake_list(const ora_scene* sc, int n, const int* ij, double* tex, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int k = 0; k < n; ++k) {
+        ora_ray o;
+        ora_trace_rk4(sc, ij[2 * k], ij[2 * k + 1], &o);
+        double* t = tex + (size_t)k * 8;
+        const bool esc = o.fate == 0;
+        /* src/metal.jl:1592-1610 */
+        for (int c = 0; c < 3; ++c) t[c] = esc ? o.dir[c] : 0.0;
+        for (int c = 0; c < 3; ++c) t[3 + c] = o.acc[c];
+        t[6] = o.alpha;
+        t[7] = esc ? 1.0 : 0.0;
+    }
+}
+
+void ora_warp_lookup(const float* m, int W, int H, const double d[3], double cphi, double sphi, double out[8]) {
+    /* src/metal.jl:3240-3299 */
+    const double rx = cphi * d[0] + sphi * d[1];
+    const double ry = -sphi * d[0] + cphi * d[1];
+    const double th = std::acos(clampd(rx, -1.0, 1.0));
+    const double ph = std::atan2(d[2], -ry);
+    const double fu = ph * H / M_PI + W / 2.0;
+    const double fv = th * H / M_PI;
+    const int i0 = (int)std::floor(fu - 0.5), j0 = (int)std::floor(fv - 0.5);
+    const double tu = fu - 0.5 - i0, tv = fv - 0.5 - j0;
+    const int ia = ((i0 % W) + W) % W, ib = (((i0 + 1) % W) + W) % W;
+    const int ja = std::min(std::max(j0, 0), H - 1), jb = std::min(std::max(j0 + 1, 0), H - 1);
+    const double a00 = (1 - tu) * (1 - tv), a10 = tu * (1 - tv), a01 = (1 - tu) * tv, a11 = tu * tv;
+    const float* p00 = m + ((size_t)ja * W + ia) * 8;
+    const float* p10 = m + ((size_t)ja * W + ib) * 8;
+    const float* p01 = m + ((size_t)jb * W + ia) * 8;
+    const float* p11 = m + ((size_t)jb * W + ib) * 8;
+    for (int c = 0; c < 8; ++c) out[c] = p00[c] * a00 + p10[c] * a10 + p01[c] * a01 + p11[c] * a11;
+    const double e1 = cphi * out[0] + sphi * out[1], e2 = -sphi * out[0] + cphi * out[1];
+    out[0] = e1;
+    out[1] = e2;
+}
+
+void ora_warp_sample(const ora_scene* sc, const float* m0, int w0, int h0, const float* m1, int w1, int h1,
+                     const double* p, int width, int height, double* out) {
+    /* src/metal.jl:3312-3372, pixel (i, j) 0-based */
+    const double half_h = height / 2.0;
+    for (int j = 0; j < height; ++j)
+        for (int i = 0; i < width; ++i) {
+            const double uu = (i + 0.5 - width / 2.0) / half_h, vv = (j + 0.5 - height / 2.0) / half_h;
+            const double dlx = uu * p[0], dly = vv * p[0];
+            const double nu = std::sqrt(dlx * dlx + dly * dly + 1.0);
+            const double cr = dlx / nu, cu = dly / nu, cf = 1.0 / nu;
+            const double d[3] = {cf * p[1] + cr * p[4] + cu * p[7], cf * p[2] + cr * p[5] + cu * p[8],
+                                 cf * p[3] + cr * p[6] + cu * p[9]};
+            double A[8], B[8];
+            ora_warp_lookup(m0, w0, h0, d, p[10], p[11], A);
+            ora_warp_lookup(m1, w1, h1, d, p[12], p[13], B);
+            const double w = p[19], om = 1.0 - w;
+            double v[8];
+            for (int c = 0; c < 8; ++c) v[c] = om * A[c] + w * B[c];
+            const double es = clampd(v[7], 0.0, 1.0);
+            const double wx = p[14] * v[0] - p[15] * v[1], wy = p[15] * v[0] + p[14] * v[1];
+            double sky[3] = {0, 0, 0};
+            if (es > 1e-3) {
+                const double vl = std::max(std::sqrt(wx * wx + wy * wy + v[2] * v[2]), 1e-20);
+                const double nx = wx / vl, ny = wy / vl, nz = v[2] / vl;
+                if (sc->sky) ora_sample_background(sc->sky, sc->sky_w, sc->sky_h, std::acos(clampd(nz, -1.0, 1.0)),
+                                                   std::atan2(ny, nx), sky);
+                if (sc->star_params[0] > 0.0 && sc->star_lut) {
+                    const double tw = sc->star_params[15];
+                    double st[3];
+                    ora_starfield(sc, nx, ny, nz, st);
+                    for (int c = 0; c < 3; ++c) sky[c] = sky[c] * tw + sc->star_params[0] * st[c];
+                }
+                for (int c = 0; c < 3; ++c) sky[c] *= es;
+            }
+            double* o = out + ((size_t)j * width + i) * 3;
+            for (int c = 0; c < 3; ++c) o[c] = (v[3 + c] + v[6] * sky[c]) * p[20];
+        }
+}
+
 }  // extern "C"

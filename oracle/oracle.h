@@ -138,6 +138,20 @@ long long ora_render(const ora_scene* sc, int method, double tol, int row0, int 
 void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const int* ij,
                     double* rec, int nthreads);
 
+/* Warp-map texels for a list of pixels (the tracer's BAKE output,
+ * src/metal.jl:1577-1611): tex[n][8] = escape direction (0 unless escaped),
+ * accumulated emission, transmittance, escape flag. */
+void ora_bake_list(const ora_scene* sc, int n, const int* ij, double* tex, int nthreads);
+/* _warp_lookup (src/metal.jl:3240-3299): bilinear sample of a canonical map
+ * float[maph][mapw][8] at world direction d, un-rotated by (cphi, sphi). */
+void ora_warp_lookup(const float* map, int mapw, int maph, const double d[3], double cphi, double sphi,
+                     double out[8]);
+/* warp_sample_kernel! (src/metal.jl:3312-3372) for every pixel of a
+ * width x height frame: out[H][W][3].  The scene supplies the sky and the
+ * starfield; p is the 24-float parameter block (0-based, as st_warp_sample). */
+void ora_warp_sample(const ora_scene* sc, const float* m0, int w0, int h0, const float* m1, int w1, int h1,
+                     const double p[24], int width, int height, double* out);
+
 /* Tsitouras 5(4) tableau as used by the oracle: c[7], a[7][7], b[7], btilde[7]. */
 void ora_tsit5_tableau(double c[7], double a[49], double b[7], double btilde[7]);
 

@@ -340,8 +340,9 @@ class HipPreviewContext:
 
     def set_starfield(self, strength=1.0, texture_weight=0.0, height=None, fov_factor=0.55, density=384, fill=0.5,
                       flux=0.011, psf_pixels=0.5, galactic=(0.0, 0.0, 1.0), concentration=3.0, temp_min=3000,
-                      temp_max=16000, seed=12345, wb_temperature=10000.0):
-        """set_starfield! (src/metal.jl:257-295)."""
+                      temp_max=16000, seed=12345, wb_temperature=10000.0) -> np.ndarray:
+        """set_starfield! (src/metal.jl:257-295); returns the star_params[16]
+        block it uploaded."""
         H = height if height is not None else self.height
         g = np.asarray(galactic, dtype=np.float64)
         gn = np.linalg.norm(g)
@@ -353,6 +354,7 @@ class HipPreviewContext:
                        temp_max - temp_min, bb.table_min, bb.table_max, bb.table_size, seed, texture_weight],
                       dtype=np.float32)
         L.check(L.lib().st_ctx_set_starfield(self._h, L.fptr(sp), L.fptr(bb.table), bb.table_size))
+        return sp
 
     def enable_records(self, on: bool = True):
         L.check(L.lib().st_ctx_enable_records(self._h, int(on)))

@@ -21,6 +21,7 @@ using st::TraceParams;
 using st::fail;
 using st::make_scene;
 using st::fill_params;
+using st::sync_temperature_scale;
 
 
 int st::fail(int code, const std::string& msg) {
@@ -130,11 +131,11 @@ int ensure_queue(st_ctx* c, size_t n) {
     return ST_OK;
 }
 
-void sync_temperature_scale(st_ctx* c, float M) {
+}  // namespace
+
+void st::sync_temperature_scale(st_ctx* c, float M) {
     if (c->T_from_M) c->Tr0 = 2.0f * M;
 }
-
-}  // namespace
 
 extern "C" {
 

@@ -27,6 +27,19 @@ int fail(int code, const std::string& msg);
                             std::string(#call) + ": " + hipGetErrorString(e_));                             \
     } while (0)
 
+// Arguments of the warp-map sampler: the two maps (8 floats per texel) and
+// the 24-float parameter block of warp_sample_kernel! (0-based: 0 fov, 1-3
+// forward, 4-6 right, 7-9 up, 10-11 cos/sin of map 0's azimuth, 12-13 map 1's,
+// 14-15 the current azimuth, 19 blend weight, 20 exposure).
+struct WarpArgs {
+    float p[24];
+    const float4* m0;
+    const float4* m1;
+    int w0, h0, w1, h1;
+    int width, height;
+    float* out;
+};
+
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -36,6 +49,13 @@ struct GraphCache {
 };
 
 }  // namespace st
+
+// A warp map: mapw x maph texels of 8 floats in device memory.
+struct st_map {
+    int device = 0;
+    int width = 0, height = 0;
+    float* buf = nullptr;
+};
 
 // The opaque context of the C ABI.
 struct st_ctx {
@@ -100,6 +120,8 @@ DeviceScene make_scene(const st_ctx* c);
 // make_scene(c) in the context's device copy (uploaded when it changed).
 int upload_scene(st_ctx* c, const DeviceScene** out);
 int fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceParams& P);
+// The reference temperature law's radius scale follows M unless set explicitly.
+void sync_temperature_scale(st_ctx* c, float M);
 
 // adapt: the in-kernel adaptive Tsit5 (order 45/46) instantiations.
 hipError_t launch_tiles(int fp64, bool kerr, bool vol, bool route, bool adapt, const DeviceScene* D,
@@ -114,6 +136,7 @@ hipError_t launch_queue64_adapt(bool kerr, bool vol, const DeviceScene* D, const
 hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F0,
                           size_t capacity, hipStream_t s);
 hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s);
+hipError_t launch_warp_sample(const DeviceScene* D, const WarpArgs& A, hipStream_t s);
 hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene* D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,
                         unsigned* work_next, int nblocks, hipStream_t s);

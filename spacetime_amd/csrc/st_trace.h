@@ -43,6 +43,7 @@ struct TraceParams {
     float promote_band, promote_horizon;
     float gas_ds;            // gas sample spacing (0: stride * dt)
     int assign;              // 1: out = sample (batched frames); 0: out += weight * sample
+    int bake;                // 1: warp-map texels (8 floats per pixel) instead of radiance
     float* out;
     float* rec;              // ST_REC_N floats per pixel, or null
     uint2* queue;            // promoted rays (frame, pixel)
@@ -66,7 +67,8 @@ struct FrameRef {
 };
 
 template <typename T> struct RayOut {
-    T rgb[3];
+    T rgb[3];                // radiance; accumulated emission alone for a warp-map texel
+    T alpha;                 // transmittance to the sky
     T dir[3];
     T ham, r_end, rk_min;
     int fate, steps, flags;
@@ -888,6 +890,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     o.r_end = rf;
     o.rk_min = A::psqrt(mx(rk_min, T(0)));
     o.ham = nonfinite ? T(NAN) : hamiltonian<T, KERR>(s, pt, M, a) / (pt * pt);
+    o.alpha = alpha;
     if (hit || captured0 || rf < T(4) * M) {
         o.fate = nonfinite ? 2 : 1;
         o.rgb[0] = acc[0]; o.rgb[1] = acc[1]; o.rgb[2] = acc[2];
@@ -907,6 +910,13 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
         dx = ix / il; dy = iy / il; dz = iz / il;
     }
     o.dir[0] = dx; o.dir[1] = dy; o.dir[2] = dz;
+    o.fate = 0;
+    if (P.bake) {
+        // warp-map texel (src/metal.jl:1577-1611): the escape direction and
+        // the emission in front of the sky, without the sky itself
+        o.rgb[0] = acc[0]; o.rgb[1] = acc[1]; o.rgb[2] = acc[2];
+        return;
+    }
     const T th = A::acos(clampv(dz, T(-1), T(1)));
     const T ph = A::atan2(dy, dx);
     T col[3];
@@ -925,7 +935,6 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
         col[1] *= T(90.2) / (A::exp(T(4.513) / K.scam) - T(1));
         col[2] *= T(206.5) / (A::exp(T(5.335) / K.scam) - T(1));
     }
-    o.fate = 0;
     o.rgb[0] = acc[0] + alpha * col[0];
     o.rgb[1] = acc[1] + alpha * col[1];
     o.rgb[2] = acc[2] + alpha * col[2];
@@ -935,6 +944,21 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
 template <typename T>
 __device__ __forceinline__ void store_ray(const TraceParams& P, const FrameRef& F, int i, int j, const RayOut<T>& o) {
     const size_t px = (size_t)j * F.width + i;
+    if (P.bake) {
+        // 8 channels (src/metal.jl:1577-1611): escape direction (0 unless
+        // escaped), premultiplied emission, transmittance, escape flag
+        float* m = F.out + px * 8;
+        const bool esc = o.fate == 0;
+        m[0] = esc ? (float)o.dir[0] : 0.0f;
+        m[1] = esc ? (float)o.dir[1] : 0.0f;
+        m[2] = esc ? (float)o.dir[2] : 0.0f;
+        m[3] = (float)o.rgb[0];
+        m[4] = (float)o.rgb[1];
+        m[5] = (float)o.rgb[2];
+        m[6] = (float)o.alpha;
+        m[7] = esc ? 1.0f : 0.0f;
+        return;
+    }
     float* out = F.out + px * 3;
     if (P.assign) {
         out[0] = (float)o.rgb[0];

@@ -1,0 +1,275 @@
+// Warp-map sampling (src/metal.jl:3245-3372): composite a frame from two baked
+// equirectangular warp maps with no geodesic integration.  Each map texel holds
+// 8 floats (escape direction, premultiplied emission, transmittance, escape
+// flag; written by the tracer's bake mode); a pixel's world direction indexes
+// both maps (stored canonically, at azimuth 0), the two bilinear samples are
+// blended, and the sky and the starfield are evaluated against the blended
+// direction at full output resolution.
+//
+// Bound: L2.  Per pixel the kernel gathers 2 maps x 4 texels x 32 B = 256 B
+// (the maps are a few MB and stay cache-resident), 4 sky texels x 16 B, and
+// writes 12 B.
+#include <hip/hip_runtime.h>
+
+#include "st_internal.h"
+#include <cstring>
+
+#include "st_trace.h"
+
+namespace st {
+
+namespace {
+
+struct Texel8 {
+    float v[8];
+};
+
+// _warp_lookup (src/metal.jl:3240-3299): bilinear sample of one canonical map
+// at world direction d, un-rotated by the map's azimuth (cphi, sphi); the
+// stored direction is rotated the same way, so two maps blend in one frame.
+__device__ __forceinline__ Texel8 warp_lookup(const float4* __restrict__ m, int W, int H, float dx, float dy, float dz,
+                                              float cphi, float sphi) {
+    const float rx = cphi * dx + sphi * dy;
+    const float ry = -sphi * dx + cphi * dy;
+    // the bake basis is R_z(phi) (x, -y, z)
+    const float th = acosf(fminf(fmaxf(rx, -1.0f), 1.0f));
+    const float ph = atan2f(dz, -ry);
+    const float fH = (float)H;
+    // (the clamps only keep a non-finite direction's indices in range)
+    const float fu = fminf(fmaxf(ph * fH / 3.1415927f + (float)W / 2.0f, -1e6f), 1e6f);
+    const float fv = fminf(fmaxf(th * fH / 3.1415927f, 0.0f), fH);
+    const int i0 = (int)floorf(fu - 0.5f);
+    const int j0 = (int)floorf(fv - 0.5f);
+    const float tu = fu - 0.5f - (float)i0;
+    const float tv = fv - 0.5f - (float)j0;
+    // phi wraps, theta clamps (the poles are points, not a seam); a map of
+    // other than 2:1 aspect can need more than one period of wrap
+    int ia = i0 % W;
+    ia += ia < 0 ? W : 0;
+    int ib = (i0 + 1) % W;
+    ib += ib < 0 ? W : 0;
+    const int ja = j0 < 0 ? 0 : (j0 > H - 1 ? H - 1 : j0);
+    const int jb = j0 + 1 < 0 ? 0 : (j0 + 1 > H - 1 ? H - 1 : j0 + 1);
+    const float a00 = (1.0f - tu) * (1.0f - tv);
+    const float a10 = tu * (1.0f - tv);
+    const float a01 = (1.0f - tu) * tv;
+    const float a11 = tu * tv;
+    const float4* p00 = m + 2 * ((size_t)ja * W + ia);
+    const float4* p10 = m + 2 * ((size_t)ja * W + ib);
+    const float4* p01 = m + 2 * ((size_t)jb * W + ia);
+    const float4* p11 = m + 2 * ((size_t)jb * W + ib);
+    const float4 x00 = p00[0], y00 = p00[1], x10 = p10[0], y10 = p10[1];
+    const float4 x01 = p01[0], y01 = p01[1], x11 = p11[0], y11 = p11[1];
+    Texel8 t;
+    t.v[0] = x00.x * a00 + x10.x * a10 + x01.x * a01 + x11.x * a11;
+    t.v[1] = x00.y * a00 + x10.y * a10 + x01.y * a01 + x11.y * a11;
+    t.v[2] = x00.z * a00 + x10.z * a10 + x01.z * a01 + x11.z * a11;
+    t.v[3] = x00.w * a00 + x10.w * a10 + x01.w * a01 + x11.w * a11;
+    t.v[4] = y00.x * a00 + y10.x * a10 + y01.x * a01 + y11.x * a11;
+    t.v[5] = y00.y * a00 + y10.y * a10 + y01.y * a01 + y11.y * a11;
+    t.v[6] = y00.z * a00 + y10.z * a10 + y01.z * a01 + y11.z * a11;
+    t.v[7] = y00.w * a00 + y10.w * a10 + y01.w * a01 + y11.w * a11;
+    const float e1 = cphi * t.v[0] + sphi * t.v[1];
+    const float e2 = -sphi * t.v[0] + cphi * t.v[1];
+    t.v[0] = e1;
+    t.v[1] = e2;
+    return t;
+}
+
+// warp_sample_kernel! (src/metal.jl:3312-3372).  One thread per pixel, 16x16
+// pixel tiles per block so neighbouring pixels gather neighbouring texels.
+__global__ __launch_bounds__(256) void k_warp_sample(const DeviceScene* __restrict__ Dp, WarpArgs A) {
+    const DeviceScene& D = *Dp;
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= A.width || j >= A.height) return;
+    const float* p = A.p;
+    const float fov = p[0];
+    const float half_h = (float)A.height / 2.0f;
+    const float uu = ((float)i + 0.5f - (float)A.width / 2.0f) / half_h;
+    const float vv = ((float)j + 0.5f - (float)A.height / 2.0f) / half_h;
+    const float dlx = uu * fov, dly = vv * fov;
+    const float nu = sqrtf(dlx * dlx + dly * dly + 1.0f);
+    const float cr = dlx / nu, cu = dly / nu, cf = 1.0f / nu;
+    // pixel direction in world coordinates through the runtime camera basis
+    const float dx = cf * p[1] + cr * p[4] + cu * p[7];
+    const float dy = cf * p[2] + cr * p[5] + cu * p[8];
+    const float dz = cf * p[3] + cr * p[6] + cu * p[9];
+    const Texel8 a = warp_lookup(A.m0, A.w0, A.h0, dx, dy, dz, p[10], p[11]);
+    const Texel8 b = warp_lookup(A.m1, A.w1, A.h1, dx, dy, dz, p[12], p[13]);
+    const float w = p[19], om = 1.0f - w;
+    const float vx = om * a.v[0] + w * b.v[0];
+    const float vy = om * a.v[1] + w * b.v[1];
+    const float vz = om * a.v[2] + w * b.v[2];
+    const float dr = om * a.v[3] + w * b.v[3];
+    const float dg = om * a.v[4] + w * b.v[4];
+    const float db = om * a.v[5] + w * b.v[5];
+    const float al = om * a.v[6] + w * b.v[6];
+    const float es = fminf(fmaxf(om * a.v[7] + w * b.v[7], 0.0f), 1.0f);
+    // back from canonical form to the ship's current azimuth, so the sky is
+    // sampled in world coordinates
+    const float cn = p[14], sn = p[15];
+    const float wx = cn * vx - sn * vy;
+    const float wy = sn * vx + cn * vy;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    if (es > 1e-3f) {
+        const float vl = fmaxf(sqrtf(wx * wx + wy * wy + vz * vz), 1e-20f);
+        const float nx = wx / vl, ny = wy / vl, nz = vz / vl;
+        const float thb = acosf(fminf(fmaxf(nz, -1.0f), 1.0f));
+        const float phb = atan2f(ny, nx);
+        float col[3];
+        sample_sky<float>(D, thb, phb, col);
+        sr = col[0]; sg = col[1]; sb = col[2];
+        if (D.star_params[0] > 0.0f) {
+            const float tw = D.star_params[15];
+            float stc[3];
+            starfield<float>(D, nx, ny, nz, stc);
+            sr = sr * tw + D.star_params[0] * stc[0];
+            sg = sg * tw + D.star_params[0] * stc[1];
+            sb = sb * tw + D.star_params[0] * stc[2];
+        }
+        sr *= es; sg *= es; sb *= es;
+    }
+    const float ex = p[20];
+    float* o = A.out + ((size_t)j * A.width + i) * 3;
+    o[0] = (dr + al * sr) * ex;
+    o[1] = (dg + al * sg) * ex;
+    o[2] = (db + al * sb) * ex;
+}
+
+}  // namespace
+
+hipError_t launch_warp_sample(const DeviceScene* D, const WarpArgs& A, hipStream_t s) {
+    const dim3 grid((A.width + 15) / 16, (A.height + 15) / 16);
+    hipLaunchKernelGGL(k_warp_sample, grid, dim3(256), 0, s, D, A);
+    return hipGetLastError();
+}
+
+}  // namespace st
+
+// ---- C ABI: warp maps ----
+using st::fail;
+
+extern "C" {
+
+int st_map_create(st_ctx* c, int mapw, int maph, st_map** out) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (mapw < 2 || maph < 2) return fail(ST_ERR_INVALID_ARG, "a warp map must be at least 2x2");
+    if ((size_t)mapw * (size_t)maph > ((size_t)1 << 28)) return fail(ST_ERR_INVALID_ARG, "warp map too large");
+    ST_HIP(hipSetDevice(c->device));
+    st_map* m = new st_map();
+    m->device = c->device;
+    m->width = mapw;
+    m->height = maph;
+    const size_t bytes = sizeof(float) * ST_MAP_CHANNELS * (size_t)mapw * maph;
+    if (hipMalloc(&m->buf, bytes) != hipSuccess) {
+        delete m;
+        return fail(ST_ERR_OOM, "hipMalloc of the warp map failed");
+    }
+    if (hipMemset(m->buf, 0, bytes) != hipSuccess) {
+        (void)hipFree(m->buf);
+        delete m;
+        return fail(ST_ERR_DEVICE, "hipMemset of the warp map failed");
+    }
+    *out = m;
+    return ST_OK;
+}
+
+int st_map_destroy(st_map* m) {
+    if (!m) return ST_OK;
+    if (m->buf) {
+        (void)hipSetDevice(m->device);
+        (void)hipFree(m->buf);
+    }
+    delete m;
+    return ST_OK;
+}
+
+int st_bake_warp_map(st_ctx* c, st_map* m, const float cam[49], const double* cam64, const float sp[8],
+                     const st_trace_args* a) {
+    if (!c || !m || !cam || !sp || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (m->device != c->device) return fail(ST_ERR_INVALID_ARG, "map and context are on different devices");
+    if (a->width != m->width || a->height != m->height)
+        return fail(ST_ERR_INVALID_ARG, "args width/height must match the map");
+    if (a->precision == ST_PREC_MIXED)
+        return fail(ST_ERR_UNSUPPORTED, "warp maps are baked in FP32 or FP64");
+    st::TraceParams P;
+    int rc = st::fill_params(c, sp, a, P);
+    if (rc != ST_OK) return rc;
+    P.bake = 1;
+    P.assign = 1;
+    st::sync_temperature_scale(c, sp[0]);
+    const st::DeviceScene* D = nullptr;
+    if ((rc = st::upload_scene(c, &D)) != ST_OK) return rc;
+    st::FrameRef F;
+    std::memset(&F, 0, sizeof(F));
+    std::memcpy(F.c32, cam, sizeof(F.c32));
+    if (cam64) {
+        std::memcpy(F.c64, cam64, sizeof(F.c64));
+        F.has64 = 1;
+    }
+    F.width = m->width;
+    F.height = m->height;
+    F.out = m->buf;
+    F.rec = nullptr;
+    const bool kerr = sp[6] != 0.0f;
+    const bool vol = c->has_volume && c->vol_on;
+    const bool adapt = a->order == 45 || a->order == 46;
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
+    ST_HIP(st::launch_tiles(a->precision == ST_PREC_FP64, kerr, vol, false, adapt, D, P, F, 0, 0, m->width, m->height,
+                            8, c->stream));
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = false;
+    c->have_last = true;
+    return ST_OK;
+}
+
+int st_map_download(st_ctx* c, const st_map* m, float* host, size_t n) {
+    if (!c || !m || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (n != (size_t)ST_MAP_CHANNELS * m->width * m->height)
+        return fail(ST_ERR_INVALID_ARG, "nfloats must be 8 * mapw * maph");
+    ST_HIP(hipMemcpyAsync(host, m->buf, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_map_upload(st_ctx* c, st_map* m, const float* host, size_t n) {
+    if (!c || !m || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (n != (size_t)ST_MAP_CHANNELS * m->width * m->height)
+        return fail(ST_ERR_INVALID_ARG, "nfloats must be 8 * mapw * maph");
+    ST_HIP(hipMemcpyAsync(m->buf, host, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_warp_sample(st_ctx* c, const st_map* m0, const st_map* m1, const float params[24]) {
+    if (!c || !m0 || !m1 || !params) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (m0->device != c->device || m1->device != c->device)
+        return fail(ST_ERR_INVALID_ARG, "maps and context are on different devices");
+    const st::DeviceScene* D = nullptr;
+    int rc = st::upload_scene(c, &D);
+    if (rc != ST_OK) return rc;
+    st::WarpArgs A;
+    std::memcpy(A.p, params, sizeof(A.p));
+    A.m0 = reinterpret_cast<const float4*>(m0->buf);
+    A.m1 = reinterpret_cast<const float4*>(m1->buf);
+    A.w0 = m0->width; A.h0 = m0->height;
+    A.w1 = m1->width; A.h1 = m1->height;
+    A.width = c->width;
+    A.height = c->height;
+    A.out = c->out;
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
+    ST_HIP(st::launch_warp_sample(D, A, c->stream));
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = false;
+    c->have_last = true;
+    return ST_OK;
+}
+
+}  // extern "C"

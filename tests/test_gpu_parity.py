@@ -296,6 +296,38 @@ def test_raygen_variants_fp64_vs_oracle(hip, oracle_lib, case):
         assert np.percentile(ang, 99.9) < 1e-6 and ang.max() < 2e-5
 
 
+def test_starfield_fp64_vs_oracle(hip, oracle_lib):
+    """The procedural starfield (starfield_mtl, src/metal.jl:652-724, with
+    set_starfield!'s parameters, :257-295) on escaping rays: FP64 kernel vs
+    the oracle's FP64 restatement with its own star LUT, texture weight 0.3."""
+    W, H = 160, 90
+    sc = scenes.kerr09_1080p(W, H)
+    print("case: starfield fp64 160x90", flush=True)
+    ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=sc.disc)
+    sp = ctx.set_starfield(strength=1.0, texture_weight=0.3, height=H, density=2048)
+    ctx.enable_records(True)
+    c32, c64 = sc.cam_params()
+    ctx.clear()
+    ctx.trace(c32, c64, sc.spacetime_params(), api.trace_args(W, H, sc.rule, precision=L.ST_PREC_FP64))
+    rec = ctx.records().reshape(-1, L.REC_N).astype(np.float64)
+    ctx.close()
+    slut = O.blackbody_lut(10000.0, float(sp[11]), float(sp[12]), int(sp[13]))
+    o = oracle_scene(sc, stars=(sp, slut))
+    ij = np.array([(i, j) for j in range(H) for i in range(W)], dtype=np.int32)
+    orec = o.trace_list(ij, "rk4")
+    same = (rec[:, 0] == orec[:, 0]) & (rec[:, 1] == orec[:, 1])
+    esc = same & (orec[:, 0] == 0)
+    # the stars change the escaped pixels: the oracle without them differs
+    plain = oracle_scene(sc).trace_list(ij, "rk4")
+    lift = np.abs(orec[esc, 5:8] - plain[esc, 5:8]).max(axis=1) / np.maximum(np.abs(plain[esc, 5:8]).max(axis=1), 1e-6)
+    rel = np.max(np.abs(rec[esc, 5:8] - orec[esc, 5:8]), axis=1) / np.maximum(np.max(np.abs(orec[esc, 5:8]), axis=1),
+                                                                              1e-6)
+    print(f"identical fate+steps {same.mean():.5f}; pixels the stars change by >1%: {np.mean(lift > 0.01):.3f}; "
+          f"radiance rel p99.9 {np.percentile(rel, 99.9):.2e} max {rel.max():.2e}")
+    assert same.mean() >= 0.995 and np.mean(lift > 0.01) > 0.05
+    assert np.mean(rel <= 1e-5) >= 0.999 and rel.max() <= 1e-3
+
+
 def test_nonfinite_camera_paints_black(hip):
     """NaN in the camera block: every ray is non-finite, painted black, fate 2
     — a pixel value, never an error (src/metal.jl:1414-1421)."""
