@@ -340,11 +340,12 @@ def bench_warp(args):
     ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=sc.disc)
     ctx.set_starfield(height=H)
     p0, p1 = np.array([0.0, -30.0, 3.0]), np.array([6.0, -29.0, 2.5])
-    bake_ms = []
     m0, phi0 = warp.bake_warp_map(ctx, sc.spacetime, p0, rule=sc.rule)
-    bake_ms.append(ctx.stats()["ms_total"])
     m1, phi1 = warp.bake_warp_map(ctx, sc.spacetime, p1, rule=sc.rule)
-    bake_ms.append(ctx.stats()["ms_total"])
+    bake_ms = []
+    for _ in range(3):      # re-bake into the same maps: steady-state bake time
+        warp.bake_warp_map(ctx, sc.spacetime, p0, rule=sc.rule, out=m0)
+        bake_ms.append(ctx.stats()["ms_total"])
     cam = api.Camera(0.7 * p0 + 0.3 * p1, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0))
     prm = warp.warp_params(cam, phi0, phi1, 0.3)
 
@@ -381,8 +382,10 @@ def bench_warp(args):
                                 "maps and the sky are cache-resident, so this is an L2 rate"}
     line["live_trace"] = {"kernel_ms": round(live_ms, 4), "speedup_of_warp": round(live_ms / kms, 1),
                           "note": "plain FP32 RK4 trace of the same camera's frame, same rule and scene"}
-    line["bake"] = {"texels": 512 * 256, "ms_per_map": round(float(np.mean(bake_ms)), 4),
-                    "Mrays_per_s": round(512 * 256 / (float(np.mean(bake_ms)) * 1e-3) / 1e6, 1)}
+    line["bake"] = {"texels": 512 * 256, "ms_per_map": round(float(np.median(bake_ms)), 4),
+                    "Mrays_per_s": round(512 * 256 / (float(np.median(bake_ms)) * 1e-3) / 1e6, 1),
+                    "note": "one 512x256 equirect map of the Kerr a=0.9 scene, FP32 RK4 (131k rays fill a "
+                            "twentieth of the chip)"}
     line["cpu_baseline"] = None
     for m in (m0, m1):
         m.close()
