@@ -96,7 +96,7 @@ typedef struct {
     int precision;         /* ST_PREC_* */
     float promote_band;    /* FP64 promotion: |r_peri - r_ph| < band (M units) */
     float promote_horizon; /* FP64 promotion: KS r < promote_horizon * r_plus */
-    int stat;              /* reserved: refinement statistics variant */
+    int stat;              /* 1: also accumulate refinement moments (st_trace only; st_download_stat) */
     float gas_ds;          /* gas sample spacing in arc length (0: the reference's stride * dt) */
 } st_trace_args;
 
@@ -157,6 +157,15 @@ int st_sync(st_ctx* ctx);
 /* Copy the (3, W, H) buffer to host float[H][W][3] (_download_rgb,
  * src/metal.jl:1991-1998; copyto!(host, out_gpu), src/metal.jl:1813). */
 int st_download(st_ctx* ctx, float* host_out);
+/* Adaptive-refinement moments (render_draft_mtl's STAT pass,
+ * src/metal.jl:1639-1648, 1695-1702, 2110-2114): a trace with args->stat = 1
+ * also adds, per pixel, weight * L, weight * L^2 (L the sample's Rec. 709
+ * luminance) and weight if the Bardeen launch test (args->bardeen, on by
+ * default) flags the ray as grazing the photon shell, into the context's
+ * (3, W, H) stat buffer (allocated at
+ * the first such trace, zeroed then and by st_clear).  st_trace only.  Copy
+ * it to host float[H][W][3]: */
+int st_download_stat(st_ctx* ctx, float* host_out);
 /* Copy the per-pixel records to host float[H][W][ST_REC_N]. */
 int st_download_records(st_ctx* ctx, float* host_rec);
 /* Counters and device timings of the last trace call (synchronises). */

@@ -386,6 +386,13 @@ class HipPreviewContext:
         L.check(L.lib().st_download(self._h, L.fptr(out)))
         return out
 
+    def stat(self) -> np.ndarray:
+        """The refinement moments of STAT traces: float32 (H, W, 3) = sum w L,
+        sum w L^2, graze weight."""
+        out = np.empty((self.height, self.width, 3), dtype=np.float32)
+        L.check(L.lib().st_download_stat(self._h, L.fptr(out)))
+        return out
+
     def records(self) -> np.ndarray:
         out = np.empty((self.height, self.width, L.REC_N), dtype=np.float32)
         L.check(L.lib().st_download_records(self._h, L.fptr(out)))
@@ -434,10 +441,18 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
                      progress=None, fisheye_deg: float = 0.0, aperture_world: float = 0.0,
                      focus_dist: float = 1.0, relativistic: bool = False, pixel_jitter: bool = True,
                      order: int = 4, tol: float = 1e-4, rule: Optional[StepRule] = None,
-                     precision: int = L.ST_PREC_FP32) -> np.ndarray:
+                     precision: int = L.ST_PREC_FP32, refine: int = 1, refine_var: float = 0.35,
+                     refine_graze: float = 0.05, refine_margin: int = 8, return_box: bool = False):
     """render_draft_mtl (src/metal.jl:2040-2242): samples^2 stratified passes,
-    per-pixel decorrelated jitter, optional thin-lens strata.  Motion blur
-    and adaptive refinement are not built."""
+    per-pixel decorrelated jitter, optional thin-lens strata, and adaptive
+    refinement: with refine > 1 the base passes also accumulate per-pixel
+    luminance moments and the photon-shell graze fraction; pixels whose
+    per-ray sigma exceeds refine_var * (mean + 0.01) or whose graze fraction
+    exceeds refine_graze are flagged, and the flagged bounding box (plus
+    refine_margin pixels) is re-rendered refine - 1 more times with fresh
+    strata, its pixels then scaled by 1/refine.  Motion blur is not built.
+    With return_box the refined box (col0, ncols, row0, nrows) or None is
+    returned as well."""
     _check_size(ctx, width, height)
     rng = rng if rng is not None else np.random.default_rng()
     if rule is None:
@@ -459,23 +474,61 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
         # jittered_grid: one random point per stratum, shared by the frame
         offs = [((a + rng.random()) / samples, (b + rng.random()) / samples) for a, b in cells]
     lens_perm = rng.permutation(samples ** 2)
+    ndispatch = len(offs) * max(refine, 1)
+    done = 0
+    pass_base = 0
+
+    def run_block(offs, lens_perm, row_a, nrows, col_a, ncols, stat):
+        # one block = samples^2 passes over a rectangle; `pass` stays
+        # globally unique so every per-pixel hash seed differs across blocks
+        nonlocal done, pass_base
+        for p, (du, dv) in enumerate(offs):
+            pss = pass_base + p + 1
+            cp32, cp64 = c32.copy(), c64.copy()
+            if use_dof:
+                m = int(lens_perm[p])
+                for arr in (cp32, cp64):
+                    arr[22] = (m // samples) / samples
+                    arr[23] = (m % samples) / samples
+                    arr[25] = 1.0 / samples
+                    arr[26] = aperture_world / 2.0
+                    arr[27] = pss
+            a = trace_args(width, height, rule, precision=precision, ju=du, jv=dv, jw=jw, jseed=15485863 + pss,
+                           weight=weight, row0=row_a, rows=nrows, col0=col_a, cols=ncols)
+            a.stat = 1 if stat else 0
+            ctx.trace(cp32, cp64, sp, a)
+            done += 1
+            if progress is not None:
+                ctx.sync()
+                progress(min(done / ndispatch, 1.0))
+        pass_base += len(offs)
+
     ctx.clear()
-    for p, (du, dv) in enumerate(offs):
-        cp32, cp64 = c32.copy(), c64.copy()
-        if use_dof:
-            m = int(lens_perm[p])
-            for arr in (cp32, cp64):
-                arr[22] = (m // samples) / samples
-                arr[23] = (m % samples) / samples
-                arr[25] = 1.0 / samples
-                arr[26] = aperture_world / 2.0
-                arr[27] = p + 1
-        ctx.trace(cp32, cp64, sp, trace_args(width, height, rule, precision=precision, ju=du, jv=dv, jw=jw,
-                                             jseed=15485863 + p + 1, weight=weight))
-        if progress is not None:
-            ctx.sync()
-            progress((p + 1) / len(offs))
-    return ctx.download()
+    run_block(offs, lens_perm, 0, height, 0, width, refine > 1)
+    box = None
+    if refine > 1:
+        st = ctx.stat()
+        m1 = st[..., 0]
+        s2 = st[..., 1] - m1 * m1
+        flag = ((s2 > 0) & (np.sqrt(np.maximum(s2, 0)) > refine_var * (m1 + 0.01))) | (st[..., 2] > refine_graze)
+        if flag.any():
+            jj, ii = np.nonzero(flag)
+            c_a = max(int(ii.min()) - refine_margin, 0)
+            ncols_r = min(int(ii.max()) + 1 + refine_margin, width) - c_a
+            r_a = max(int(jj.min()) - refine_margin, 0)
+            nrows_r = min(int(jj.max()) + 1 + refine_margin, height) - r_a
+            for _ in range(2, refine + 1):
+                if pixel_jitter:
+                    offs_r = [(cells[k][0] / samples, cells[k][1] / samples) for k in rng.permutation(len(cells))]
+                else:
+                    offs_r = [((a + rng.random()) / samples, (b + rng.random()) / samples) for a, b in cells]
+                run_block(offs_r, rng.permutation(samples ** 2), r_a, nrows_r, c_a, ncols_r, False)
+            box = (c_a, ncols_r, r_a, nrows_r)
+    img = ctx.download()
+    if box is not None:
+        c_a, ncols_r, r_a, nrows_r = box
+        img[r_a:r_a + nrows_r, c_a:c_a + ncols_r] *= np.float32(1.0 / refine)
+    return (img, box) if return_box else img
 
 
 def render(cam: Camera, spacetime, background: np.ndarray, *, disc: Optional[AccretionDisc] = None,

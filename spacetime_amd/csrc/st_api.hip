@@ -103,7 +103,7 @@ int st::fill_params(st_ctx* c, const float sp[8], const st_trace_args* a, TraceP
         return fail(ST_ERR_UNSUPPORTED, "order must be 4, 2, 1 (fixed step) or 45, 46 (adaptive Tsit5)");
     if ((a->order == 45 || a->order == 46) && !(a->tol > 0.0f))
         return fail(ST_ERR_INVALID_ARG, "adaptive Tsit5 needs tol > 0");
-    if (a->stat) return fail(ST_ERR_UNSUPPORTED, "STAT refinement variant not built");
+    if (a->stat) return fail(ST_ERR_UNSUPPORTED, "STAT moments are accumulated by st_trace only");
     if (a->precision < 0 || a->precision > 2) return fail(ST_ERR_INVALID_ARG, "precision must be 0, 1 or 2");
     std::memset(&P, 0, sizeof(P));
     P.M = sp[0]; P.r_escape = sp[2]; P.relativistic = sp[3] > 0.5f;
@@ -212,7 +212,7 @@ int st_ctx_destroy(st_ctx* c) {
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
-    free_dev(c->out); free_dev(c->rec); free_dev(c->queue); free_dev(c->qcount);
+    free_dev(c->out); free_dev(c->rec); free_dev(c->stat); free_dev(c->queue); free_dev(c->qcount);
     free_dev(c->counters); free_dev(c->work_next); free_dev(c->frames_dev);
     free_dev(c->scene_dev);
     free_dev(c->batch_out); free_dev(c->tile_base_dev);
@@ -315,6 +315,16 @@ int st_ctx_enable_records(st_ctx* c, int on) {
 int st_clear(st_ctx* c) {
     if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
     ST_HIP(hipMemsetAsync(c->out, 0, sizeof(float) * 3 * (size_t)c->width * c->height, c->stream));
+    if (c->stat) ST_HIP(hipMemsetAsync(c->stat, 0, sizeof(float) * 3 * (size_t)c->width * c->height, c->stream));
+    return ST_OK;
+}
+
+int st_download_stat(st_ctx* c, float* host) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->stat) return fail(ST_ERR_INVALID_ARG, "no STAT trace has run on this context");
+    ST_HIP(hipMemcpyAsync(host, c->stat, sizeof(float) * 3 * (size_t)c->width * c->height, hipMemcpyDeviceToHost,
+                          c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
     return ST_OK;
 }
 
@@ -326,8 +336,20 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     if (a->row0 < 0 || a->rows < 0 || a->row0 + a->rows > c->height || a->col0 < 0 || a->col0 + cols > c->width)
         return fail(ST_ERR_INVALID_ARG, "tile outside the frame");
     TraceParams P;
-    int rc = fill_params(c, sp, a, P);
+    st_trace_args a_nostat = *a;
+    a_nostat.stat = 0;
+    int rc = fill_params(c, sp, &a_nostat, P);
     if (rc != ST_OK) return rc;
+    if (a->stat) {
+        // render_draft_mtl's stat buffer (src/metal.jl:2110-2114), zeroed
+        // when allocated and by st_clear
+        if (!c->stat) {
+            const size_t n = 3 * (size_t)c->width * c->height;
+            ST_HIP(hipMalloc(&c->stat, sizeof(float) * n));
+            ST_HIP(hipMemsetAsync(c->stat, 0, sizeof(float) * n, c->stream));
+        }
+        P.stat = c->stat;
+    }
     sync_temperature_scale(c, sp[0]);
     const DeviceScene* D = nullptr;
     if ((rc = upload_scene(c, &D)) != ST_OK) return rc;

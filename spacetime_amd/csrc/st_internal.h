@@ -80,6 +80,7 @@ struct st_ctx {
     float star_params[16] = {0};
     float* out = nullptr;
     float* rec = nullptr;
+    float* stat = nullptr;    // (3, W, H) refinement moments, allocated at the first STAT trace
     bool rec_on = false;
     uint2* queue = nullptr;
     size_t queue_cap = 0;

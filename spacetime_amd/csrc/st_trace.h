@@ -44,6 +44,7 @@ struct TraceParams {
     float gas_ds;            // gas sample spacing (0: stride * dt)
     int assign;              // 1: out = sample (batched frames); 0: out += weight * sample
     int bake;                // 1: warp-map texels (8 floats per pixel) instead of radiance
+    float* stat;             // per-pixel luminance moments + graze weight (3 floats), or null
     float* out;
     float* rec;              // ST_REC_N floats per pixel, or null
     uint2* queue;            // promoted rays (frame, pixel)
@@ -960,6 +961,17 @@ __device__ __forceinline__ void store_ray(const TraceParams& P, const FrameRef& 
         return;
     }
     float* out = F.out + px * 3;
+    if (P.stat) {
+        // adaptive-refinement moments (src/metal.jl:1639-1648, 1695-1702):
+        // sum w L, sum w L^2 and the weight of rays the Bardeen launch test
+        // (on by default) flags as grazing the photon shell
+        const T Ls = T(0.2126) * o.rgb[0] + T(0.7152) * o.rgb[1] + T(0.0722) * o.rgb[2];
+        const T w = T(P.weight);
+        float* st = P.stat + px * 3;
+        st[0] += (float)(w * Ls);
+        st[1] += (float)(w * Ls * Ls);
+        if (o.flags & 4) st[2] += (float)w;
+    }
     if (P.assign) {
         out[0] = (float)o.rgb[0];
         out[1] = (float)o.rgb[1];
