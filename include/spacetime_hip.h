@@ -166,6 +166,18 @@ int st_download(st_ctx* ctx, float* host_out);
  * the first such trace, zeroed then and by st_clear).  st_trace only.  Copy
  * it to host float[H][W][3]: */
 int st_download_stat(st_ctx* ctx, float* host_out);
+/* Depth-bucketed trace (render_depth_mtl's NB variant, src/metal.jl:1934-1988
+ * and the kernel's NB blocks :1170-1175, 1423-1427, 1487-1496, 1556-1565,
+ * 1687-1690): like st_trace, but each emission event's radiance is added to
+ * one of nbuckets (3..64) buckets by the ray's integrated coordinate path
+ * length (log-spaced over [1.5M, 120M] in nbuckets - 1 buckets), and the
+ * escaped background (transmittance x sky) to the last, into the context's
+ * (3 * nbuckets, W, H) depth buffer (allocated, and zeroed, when nbuckets
+ * changes; zeroed by st_clear).  FP32 or FP64, fixed-step orders. */
+int st_trace_depth(st_ctx* ctx, const float cam_params[49], const double* cam_params64,
+                   const float spacetime_params[8], const st_trace_args* args, int nbuckets);
+/* Copy the depth buffer to host float[H][W][3 * nbuckets]. */
+int st_download_depth(st_ctx* ctx, float* host_out, size_t nfloats);
 /* Copy the per-pixel records to host float[H][W][ST_REC_N]. */
 int st_download_records(st_ctx* ctx, float* host_rec);
 /* Counters and device timings of the last trace call (synchronises). */

@@ -76,6 +76,7 @@ _SIG = {
                                   C.c_int]),
     "ora_trace_list": (None, [C.POINTER(OraScene), C.c_int, C.c_double, C.c_int, _I, _D, C.c_int]),
     "ora_bake_list": (None, [C.POINTER(OraScene), C.c_int, _I, _D, C.c_int]),
+    "ora_depth_list": (None, [C.POINTER(OraScene), C.c_int, _I, C.c_int, _D, C.c_int]),
     "ora_warp_lookup": (None, [_F, C.c_int, C.c_int, _D, C.c_double, C.c_double, _D]),
     "ora_warp_sample": (None, [C.POINTER(OraScene), _F, C.c_int, C.c_int, _F, C.c_int, C.c_int, _D, C.c_int,
                                C.c_int, _D]),
@@ -274,6 +275,13 @@ class Scene:
         tex = np.zeros((ij.shape[0], 8))
         lib().ora_bake_list(self.ptr, ij.shape[0], ij.ctypes.data_as(_I), tex.ctypes.data_as(_D), nthreads)
         return tex
+
+    def depth_list(self, ij: np.ndarray, nb: int, nthreads=0) -> np.ndarray:
+        """Depth-bucketed samples (n, nb, 3) of the listed pixels."""
+        ij = np.ascontiguousarray(ij, dtype=np.int32)
+        out = np.zeros((ij.shape[0], nb, 3))
+        lib().ora_depth_list(self.ptr, ij.shape[0], ij.ctypes.data_as(_I), int(nb), out.ctypes.data_as(_D), nthreads)
+        return out
 
     def warp_sample(self, m0: np.ndarray, m1: np.ndarray, p, width: int, height: int) -> np.ndarray:
         """warp_sample_kernel! over a width x height frame from two maps

@@ -624,7 +624,29 @@ struct Shade {
     double alpha = 1.0;
     int ncross = 0;
     double march_acc = 0.0;
+    /* depth buckets (src/metal.jl:1170-1175): emission binned by the
+     * integrated path length ell into buckets[3 * nb] */
+    int nb = 0;
+    double ell = 0.0;
+    double* buckets = nullptr;
 };
+
+/* Bucket of the current path length, src/metal.jl:1489-1493 (0-based). */
+inline double* depth_bucket(Shade& sh) {
+    const double bscale = sh.nb > 2 ? (sh.nb - 2) * 0.22820 : 1.0;
+    int bi = 0;
+    if (sh.ell > 1.5) bi = std::min(std::max((int)((std::log(sh.ell) - 0.405465) * bscale), 0), sh.nb - 2);
+    return sh.buckets + 3 * bi;
+}
+
+inline void emit(Shade& sh, double w, const double col[3]) {
+    if (sh.nb > 0) {
+        double* b = depth_bucket(sh);
+        for (int c = 0; c < 3; ++c) b[c] += w * col[c];
+    } else {
+        for (int c = 0; c < 3; ++c) sh.acc[c] += w * col[c];
+    }
+}
 
 inline double temp_law(const ora_scene* sc, double s, double r_isco) {
     const double sT = sc->T_hold_isco ? std::max(s, r_isco) : s;
@@ -662,7 +684,7 @@ void shade_disc(const ora_scene* sc, double xh, double yh, double pxh, double py
                                         : std::exp(sc->disc_falloff * std::log(density));
     const double opacity = iscotaper * outertaper * dpow * plunge;
     const double w = sh.alpha * opacity * inten;
-    for (int c = 0; c < 3; ++c) sh.acc[c] += w * col[c];
+    emit(sh, w, col);
     sh.alpha *= (1.0 - opacity);
     sh.ncross++;
 }
@@ -687,7 +709,7 @@ void shade_gas(const ora_scene* sc, double sx, double sy, double sz, double psx,
     const double tau = sc->vol_params[7] * rho * ds_gas;
     const double aa = 1.0 - std::exp(-tau);
     const double w = sh.alpha * aa * inten * sc->vol_params[6];
-    for (int c = 0; c < 3; ++c) sh.acc[c] += w * col[c];
+    emit(sh, w, col);
     sh.alpha *= (1.0 - aa);
 }
 
@@ -759,6 +781,9 @@ void finish(const ora_scene* sc, const double s[6], double p_t, double scam, boo
     }
     o->fate = 0;
     for (int c = 0; c < 3; ++c) o->rgb[c] = sh.acc[c] + sh.alpha * col[c];
+    /* the background lives in the last (infinity) bucket, src/metal.jl:1687-1690 */
+    if (sh.nb > 0)
+        for (int c = 0; c < 3; ++c) sh.buckets[3 * (sh.nb - 1) + c] += sh.alpha * col[c];
 }
 
 inline double ks_r2(const ora_scene* sc, const double s[6]) {
@@ -822,13 +847,24 @@ extern "C" {
  * rule), 1382-1401 (RK4; midpoint and Euler for order 2 and 1), 1278-1381
  * (adaptive Tsit5, order 45/46), 1414-1421 (NaN bail), 1440-1571 (gas, thin
  * disc), 1629-1704 (fate, sky). */
-int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
+static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets);
+
+int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) { return trace_rk4(sc, i, j, o, 0, nullptr); }
+
+int ora_trace_depth(const ora_scene* sc, int i, int j, int nb, double* buckets, ora_ray* o) {
+    for (int k = 0; k < 3 * nb; ++k) buckets[k] = 0.0;
+    return trace_rk4(sc, i, j, o, nb, buckets);
+}
+
+static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets) {
     RayStart rs;
     ray_start(sc, i, j, o, rs);
     const double M = sc->M;
     double s[6];
     std::memcpy(s, rs.s, sizeof(s));
     Shade sh;
+    sh.nb = nb;
+    sh.buckets = buckets;
     bool hit = rs.no_sky, nonfinite = false, escaped_loop = false;
     double r_prev = -1.0;
     double rk2_prev = rs.rk2_prev0;
@@ -977,6 +1013,7 @@ int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
         if (!(s[0] == s[0]) || !(s[2] == s[2]) || !(s[3] == s[3]) ||
             !(std::fabs(s[0]) + std::fabs(s[1]) + std::fabs(s[2]) < 1e30)) { hit = true; nonfinite = true; break; }
         const double vlen = std::max(std::sqrt(k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]), 1e-20);
+        if (sh.nb > 0) sh.ell += h_used * vlen;   /* src/metal.jl:1423-1427 */
         if (vol && sh.alpha > 0.003 && std::min(r2, s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) < vol_rb2) {
             march_gas(sc, sp, s, h_used * vlen, rs.p_t, rs.scam, rs.isco, ds_gas, sh);
             if (sh.alpha < 0.003) break;
@@ -1240,6 +1277,17 @@ void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const in
         r[5] = o.rgb[0]; r[6] = o.rgb[1]; r[7] = o.rgb[2];
         r[8] = o.ham; r[9] = o.r_end; r[10] = o.rk_min; r[11] = o.ncross;
         r[12] = o.captured0; r[13] = o.graze0; r[14] = o.rc; r[15] = o.Rc;
+    }
+}
+
+void ora_depth_list(const ora_scene* sc, int n, const int* ij, int nb, double* out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int k = 0; k < n; ++k) {
+        ora_ray o;
+        ora_trace_depth(sc, ij[2 * k], ij[2 * k + 1], nb, out + (size_t)k * 3 * nb, &o);
     }
 }
 

@@ -138,6 +138,13 @@ long long ora_render(const ora_scene* sc, int method, double tol, int row0, int 
 void ora_trace_list(const ora_scene* sc, int method, double tol, int n, const int* ij,
                     double* rec, int nthreads);
 
+/* Depth-bucketed trace (render_depth_mtl's NB variant, src/metal.jl:1170-1175,
+ * 1423-1427, 1487-1496, 1556-1565, 1687-1690): emission binned by integrated
+ * path length ell into nb log-spaced buckets over [1.5M, 120M], the escaped
+ * background (transmittance x sky) in the last; buckets[3 * nb], zeroed here. */
+int ora_trace_depth(const ora_scene* sc, int i, int j, int nb, double* buckets, ora_ray* out);
+/* The same for a list of pixels: out[n][3 * nb]. */
+void ora_depth_list(const ora_scene* sc, int n, const int* ij, int nb, double* out, int nthreads);
 /* Warp-map texels for a list of pixels (the tracer's BAKE output,
  * src/metal.jl:1577-1611): tex[n][8] = escape direction (0 unless escaped),
  * accumulated emission, transmittance, escape flag. */

@@ -99,6 +99,8 @@ SIGNATURES = {
     "st_trace_batch_graph": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),
     "st_download_batch": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
     "st_download_stat": (C.c_int, [C.c_void_p, _F]),
+    "st_trace_depth": (C.c_int, [C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs), C.c_int]),
+    "st_download_depth": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
     "st_map_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "st_map_destroy": (C.c_int, [C.c_void_p]),
     "st_bake_warp_map": (C.c_int, [C.c_void_p, C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs)]),

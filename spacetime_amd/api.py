@@ -386,6 +386,12 @@ class HipPreviewContext:
         L.check(L.lib().st_download(self._h, L.fptr(out)))
         return out
 
+    def depth(self, nbuckets: int) -> np.ndarray:
+        """The depth-bucketed accumulation: float32 (H, W, 3 * nbuckets)."""
+        out = np.empty((self.height, self.width, 3 * nbuckets), dtype=np.float32)
+        L.check(L.lib().st_download_depth(self._h, L.fptr(out), out.size))
+        return out
+
     def stat(self) -> np.ndarray:
         """The refinement moments of STAT traces: float32 (H, W, 3) = sum w L,
         sum w L^2, graze weight."""
@@ -529,6 +535,37 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
         c_a, ncols_r, r_a, nrows_r = box
         img[r_a:r_a + nrows_r, c_a:c_a + ncols_r] *= np.float32(1.0 / refine)
     return (img, box) if return_box else img
+
+
+def render_depth_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: int = 1920, height: int = 1080,
+                     samples: int = 2, dt: float = 0.02, nbuckets: int = 10, fisheye_deg: float = 0.0,
+                     rng: Optional[np.random.Generator] = None, relativistic: bool = False,
+                     rule: Optional[StepRule] = None, precision: int = L.ST_PREC_FP32) -> np.ndarray:
+    """render_depth_mtl (src/metal.jl:1934-1988): a pinhole draft render with
+    the emission separated into `nbuckets` path-length buckets (log-spaced
+    over 1.5M-120M; the last holds the escaped background).  Returns float32
+    (H, W, 3 * nbuckets) — the (3 * nbuckets, W, H) array's bytes — for
+    depth-of-field as a post operation."""
+    if nbuckets < 3:
+        raise ValueError("nbuckets must be >= 3")
+    _check_size(ctx, width, height)
+    rng = rng if rng is not None else np.random.default_rng()
+    M = spacetime.M
+    if rule is None:
+        rule = StepRule(dt=dt, r_escape_factor=ctx.r_escape_factor)
+        r_esc = rule.escape_radius(cam, M)
+        rule.nmax = int(min(max(ctx.nmax, math.ceil((75.0 + 6.5 * math.log(r_esc / M)) * M / dt)), 40000))
+    c32, c64 = ks_cam_params(cam, M, spin(spacetime), fisheye_deg=fisheye_deg)
+    sp = spacetime_params(spacetime, cam, rule, relativistic)
+    weight = 1.0 / samples ** 2
+    # jittered_grid: one random point per stratum
+    offs = [((a + rng.random()) / samples, (b + rng.random()) / samples)
+            for b in range(samples) for a in range(samples)]
+    ctx.clear()
+    for du, dv in offs:
+        a = trace_args(width, height, rule, precision=precision, ju=du, jv=dv, weight=weight)
+        L.check(L.lib().st_trace_depth(ctx.handle, L.fptr(c32), L.dptr(c64), L.fptr(sp), C.byref(a), int(nbuckets)))
+    return ctx.depth(nbuckets)
 
 
 def render(cam: Camera, spacetime, background: np.ndarray, *, disc: Optional[AccretionDisc] = None,

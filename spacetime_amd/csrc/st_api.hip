@@ -212,7 +212,8 @@ int st_ctx_destroy(st_ctx* c) {
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
-    free_dev(c->out); free_dev(c->rec); free_dev(c->stat); free_dev(c->queue); free_dev(c->qcount);
+    free_dev(c->out); free_dev(c->rec); free_dev(c->stat); free_dev(c->depth); free_dev(c->queue);
+    free_dev(c->qcount);
     free_dev(c->counters); free_dev(c->work_next); free_dev(c->frames_dev);
     free_dev(c->scene_dev);
     free_dev(c->batch_out); free_dev(c->tile_base_dev);
@@ -316,6 +317,68 @@ int st_clear(st_ctx* c) {
     if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
     ST_HIP(hipMemsetAsync(c->out, 0, sizeof(float) * 3 * (size_t)c->width * c->height, c->stream));
     if (c->stat) ST_HIP(hipMemsetAsync(c->stat, 0, sizeof(float) * 3 * (size_t)c->width * c->height, c->stream));
+    if (c->depth)
+        ST_HIP(hipMemsetAsync(c->depth, 0, sizeof(float) * 3 * (size_t)c->depth_nb * c->width * c->height, c->stream));
+    return ST_OK;
+}
+
+int st_trace_depth(st_ctx* c, const float cam[49], const double* cam64, const float sp[8], const st_trace_args* a,
+                   int nbuckets) {
+    if (!c || !cam || !sp || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (nbuckets < 3 || nbuckets > 64) return fail(ST_ERR_INVALID_ARG, "nbuckets must be in [3, 64]");
+    if (a->width != c->width || a->height != c->height)
+        return fail(ST_ERR_INVALID_ARG, "args width/height must match the context");
+    const int cols = a->cols < 0 ? c->width : a->cols;
+    if (a->row0 < 0 || a->rows < 0 || a->row0 + a->rows > c->height || a->col0 < 0 || a->col0 + cols > c->width)
+        return fail(ST_ERR_INVALID_ARG, "tile outside the frame");
+    if (a->precision == ST_PREC_MIXED)
+        return fail(ST_ERR_UNSUPPORTED, "depth buckets are traced in FP32 or FP64 (not the mixed policy)");
+    if (a->order == 45 || a->order == 46)
+        return fail(ST_ERR_UNSUPPORTED, "depth buckets use the fixed-step integrators");
+    TraceParams P;
+    int rc = fill_params(c, sp, a, P);
+    if (rc != ST_OK) return rc;
+    // render_depth_mtl's (3 * nbuckets, W, H) buffer (src/metal.jl:1973-1974),
+    // zeroed when (re)allocated and by st_clear
+    if (!c->depth || c->depth_nb != nbuckets) {
+        free_dev(c->depth);
+        c->depth = nullptr;
+        const size_t n = 3 * (size_t)nbuckets * c->width * c->height;
+        ST_HIP(hipMalloc(&c->depth, sizeof(float) * n));
+        ST_HIP(hipMemsetAsync(c->depth, 0, sizeof(float) * n, c->stream));
+        c->depth_nb = nbuckets;
+    }
+    P.nb = nbuckets;
+    sync_temperature_scale(c, sp[0]);
+    const DeviceScene* D = nullptr;
+    if ((rc = upload_scene(c, &D)) != ST_OK) return rc;
+    FrameRef F;
+    std::memset(&F, 0, sizeof(F));
+    std::memcpy(F.c32, cam, sizeof(F.c32));
+    if (cam64) { std::memcpy(F.c64, cam64, sizeof(F.c64)); F.has64 = 1; }
+    F.width = c->width; F.height = c->height;
+    F.out = c->depth;
+    F.rec = nullptr;
+    const bool kerr = sp[6] != 0.0f;
+    const bool vol = c->has_volume && c->vol_on;
+    ST_HIP(hipEventRecord(c->ev[0], c->stream));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, c->stream));
+    ST_HIP(st::launch_depth(a->precision == ST_PREC_FP64, kerr, vol, D, P, F, a->col0, a->row0, cols, a->rows,
+                            c->stream));
+    ST_HIP(hipEventRecord(c->ev[1], c->stream));
+    ST_HIP(hipEventRecord(c->ev[2], c->stream));
+    c->last_mixed = false;
+    c->have_last = true;
+    return ST_OK;
+}
+
+int st_download_depth(st_ctx* c, float* host, size_t n) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->depth) return fail(ST_ERR_INVALID_ARG, "no depth trace has run on this context");
+    if (n != 3 * (size_t)c->depth_nb * c->width * c->height)
+        return fail(ST_ERR_INVALID_ARG, "nfloats must be 3 * nbuckets * W * H");
+    ST_HIP(hipMemcpyAsync(host, c->depth, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
     return ST_OK;
 }
 

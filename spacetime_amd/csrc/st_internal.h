@@ -81,6 +81,8 @@ struct st_ctx {
     float* out = nullptr;
     float* rec = nullptr;
     float* stat = nullptr;    // (3, W, H) refinement moments, allocated at the first STAT trace
+    float* depth = nullptr;   // (3 * depth_nb, W, H) depth-bucketed accumulation
+    int depth_nb = 0;
     bool rec_on = false;
     uint2* queue = nullptr;
     size_t queue_cap = 0;
@@ -138,6 +140,8 @@ hipError_t launch_queue32(bool kerr, bool vol, const DeviceScene* D, const Trace
                           size_t capacity, hipStream_t s);
 hipError_t launch_reset(unsigned* qcount, unsigned long long* counters, unsigned* work_next, hipStream_t s);
 hipError_t launch_warp_sample(const DeviceScene* D, const WarpArgs& A, hipStream_t s);
+hipError_t launch_depth(int fp64, bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F,
+                        int col0, int row0, int cols, int rows, hipStream_t s);
 hipError_t launch_batch(bool kerr, bool vol, bool route, const DeviceScene* D, const TraceParams& P,
                         const FrameRef* frames_dev, const int* tile_base_dev, int nframes, int total_items, int tw,
                         unsigned* work_next, int nblocks, hipStream_t s);

@@ -1,0 +1,58 @@
+// Depth-bucketed frame kernel (render_depth_mtl's NB variant,
+// src/metal.jl:1170-1175, 1423-1427, 1487-1496, 1556-1565, 1687-1690): the
+// tracer with every emission event's radiance moved into the bucket of the
+// ray's integrated path length, the sky into the last bucket.  Its own
+// translation unit and instantiation, so the plain frame kernels keep their
+// register budget.
+#include <hip/hip_runtime.h>
+
+#include "st_kernels.h"
+
+namespace st {
+
+namespace {
+
+template <typename T, bool KERR, bool VOL>
+__global__ __launch_bounds__(256) void k_trace_depth(const DeviceScene* __restrict__ Dp, TraceParams P, FrameRef F,
+                                                     TileGeom G) {
+    const DeviceScene& D = *Dp;
+    int i, j;
+    bool valid;
+    tile_pixel(G, blockIdx.x, threadIdx.x, i, j, valid);
+    unsigned long long steps = 0;
+    if (valid) {
+        RayOut<T> o;
+        trace_ray<T, KERR, VOL, false, false, true>(D, P, F, i, j, o);
+        store_depth(P, F, i, j, o);
+        steps = (unsigned long long)o.steps;
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&P.counters[sizeof(T) == 8 ? 1 : 0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_depth(int fp64, bool kerr, bool vol, const DeviceScene* D, const TraceParams& P, const FrameRef& F,
+                        int col0, int row0, int cols, int rows, hipStream_t s) {
+    TileGeom G;
+    G.col0 = col0; G.row0 = row0; G.cols = cols; G.rows = rows; G.tw = 8;
+    G.tiles_x = (cols + 7) / 8;
+    const int nblocks = G.tiles_x * ((rows + 31) / 32);
+    if (nblocks <= 0) return hipSuccess;
+#define ST_D(TT, KK, VV) hipLaunchKernelGGL((k_trace_depth<TT, KK, VV>), dim3(nblocks), dim3(256), 0, s, D, P, F, G)
+    if (fp64) {
+        if (kerr) { if (vol) ST_D(double, true, true); else ST_D(double, true, false); }
+        else { if (vol) ST_D(double, false, true); else ST_D(double, false, false); }
+    } else {
+        if (kerr) { if (vol) ST_D(float, true, true); else ST_D(float, true, false); }
+        else { if (vol) ST_D(float, false, true); else ST_D(float, false, false); }
+    }
+#undef ST_D
+    return hipGetLastError();
+}
+
+}  // namespace st

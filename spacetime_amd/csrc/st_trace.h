@@ -45,6 +45,7 @@ struct TraceParams {
     int assign;              // 1: out = sample (batched frames); 0: out += weight * sample
     int bake;                // 1: warp-map texels (8 floats per pixel) instead of radiance
     float* stat;             // per-pixel luminance moments + graze weight (3 floats), or null
+    int nb;                  // > 0: depth buckets, out holds 3 * nb floats per pixel
     float* out;
     float* rec;              // ST_REC_N floats per pixel, or null
     uint2* queue;            // promoted rays (frame, pixel)
@@ -456,6 +457,25 @@ __device__ __forceinline__ void march_gas(const DeviceScene& D, const Kin<T>& K,
     march_acc = seg - (t0 + T(n - 1) * ds_gas);
 }
 
+// Depth bucket of path length ell (src/metal.jl:1489-1493, 0-based):
+// log-spaced over [1.5M, 120M] in nb - 1 buckets; the last is the sky's.
+// The radiance gathered in acc since the last flush goes there, weighted.
+template <typename T>
+__device__ __forceinline__ void depth_flush(float* nbo, int nb, T ell, T weight, T acc[3]) {
+    using A = Ar<T>;
+    const T bscale = nb > 2 ? T(nb - 2) * T(0.22820) : T(1);
+    int bi = 0;
+    if (ell > T(1.5)) {
+        bi = (int)((A::log(ell) - T(0.405465)) * bscale);
+        bi = bi < 0 ? 0 : (bi > nb - 2 ? nb - 2 : bi);
+    }
+    float* b = nbo + 3 * bi;
+    b[0] += (float)(weight * acc[0]);
+    b[1] += (float)(weight * acc[1]);
+    b[2] += (float)(weight * acc[2]);
+    acc[0] = acc[1] = acc[2] = T(0);
+}
+
 // Launch-time radial-potential analysis (src/metal.jl:1008-1094), for a = 0
 // as well.  Returns captured (no turning point before the horizon), graze,
 // and the outermost critical point rc of R(r) in (r_plus, r_cam) with R(rc)
@@ -530,7 +550,7 @@ __device__ __forceinline__ void bardeen(const S6<T>& s, T pt, T M, T a, bool& ca
 // promote_band, that the launch analysis finds captured, or that come within
 // promote_horizon * r_plus of the horizon are handed to the FP64 queue
 // (o.routed) instead of being finished here.
-template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT = false>
+template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT = false, bool DEPTH = false>
 __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParams& P, const FrameRef& F, int i, int j,
                                           RayOut<T>& o) {
     using A = Ar<T>;
@@ -695,6 +715,12 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     T alpha_mem = T(1), alpha_reg = T(1);
     T& alpha = VOL ? alpha_reg : alpha_mem;
     int nsamp = 0;
+    // Depth buckets (src/metal.jl:1170-1175): the integrated path length and
+    // this pixel's 3 * nb output floats; each emission event's radiance is
+    // moved from acc into the bucket of the current path length.
+    const int nb = DEPTH ? P.nb : 0;
+    T ell = T(0);
+    float* const nbo = DEPTH ? F.out + ((size_t)j * F.width + i) * (size_t)(3 * nb) : nullptr;
     T march_acc = T(0);
     T rk_min = T(1e30);
     // previous step start radius and velocity (escape-direction interpolation)
@@ -856,11 +882,13 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
             !(A::fabs(s.x) + A::fabs(s.y) + A::fabs(s.z) < T(1e30))) {
             hit = true; nonfinite = true; break;
         }
+        if constexpr (DEPTH) ell += h_used * mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
         if constexpr (VOL) {
             const T rn2 = s.x * s.x + s.y * s.y + s.z * s.z;
             if (alpha > T(0.003) && mn(r2, rn2) < vol_rb2) {
                 const T vlen = mx(A::psqrt(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z), T(1e-20));
                 march_gas(D, K, sp, s, h_used * vlen, ds_gas, march_acc, alpha, acc, nsamp);
+                if constexpr (DEPTH) depth_flush(nbo, nb, ell, T(P.weight), acc);
                 if (alpha < T(0.003)) break;
             }
         }
@@ -875,6 +903,7 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
                 shade_disc(D, K, xh, yh, sp.px + cfr * (s.px - sp.px), sp.py + cfr * (s.py - sp.py), alpha_mem,
                            acc);
                 if constexpr (VOL) alpha = alpha_mem;
+                if constexpr (DEPTH) depth_flush(nbo, nb, ell, T(P.weight), acc);
             }
         }
     }
@@ -993,6 +1022,20 @@ __device__ __forceinline__ void store_ray(const TraceParams& P, const FrameRef& 
         r[10] = (float)o.flags;
         r[11] = (float)o.rk_min;
     }
+}
+
+// Store of a depth-bucketed ray: its emission is already in its buckets;
+// the sky (transmittance x background, zero for captured rays) goes to the
+// last one (src/metal.jl:1687-1690).
+template <typename T>
+__device__ __forceinline__ void store_depth(const TraceParams& P, const FrameRef& F, int i, int j,
+                                            const RayOut<T>& o) {
+    const size_t px = (size_t)j * F.width + i;
+    float* b = F.out + px * (size_t)(3 * P.nb) + 3 * (P.nb - 1);
+    const T w = T(P.weight);
+    b[0] += (float)(w * o.rgb[0]);
+    b[1] += (float)(w * o.rgb[1]);
+    b[2] += (float)(w * o.rgb[2]);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
