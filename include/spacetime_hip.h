@@ -134,6 +134,9 @@ int st_ctx_set_volume(st_ctx* ctx, const float* vol, int nr, int nphi, int nz,
                       const float vol_params[9]);
 /* set_volume_enabled! (src/metal.jl:192-196) */
 int st_ctx_set_volume_enabled(st_ctx* ctx, int enabled);
+/* Copy the context's gas texture (as uploaded, or as the fluid simulation
+ * last wrote it) to host float[nz][nphi][nr]. */
+int st_ctx_download_volume(st_ctx* ctx, float* host_out, size_t nfloats);
 /* set_march_stride! (src/metal.jl:304-308) */
 int st_ctx_set_march_stride(st_ctx* ctx, int stride);
 /* set_starfield! (src/metal.jl:257-295).  lut: float[n][3] star colour LUT. */
@@ -213,6 +216,34 @@ int st_map_upload(st_ctx* ctx, st_map* map, const float* host, size_t nfloats);
  * context's sky and starfield are sampled against the blended direction.
  * Asynchronous on the context's stream. */
 int st_warp_sample(st_ctx* ctx, const st_map* map0, const st_map* map1, const float params[24]);
+
+/* ---- live fluid-disc simulation (src/disc_sim.jl) ----
+ * A 2D stable-fluids solver on the gas volume's (log s, phi) grid (Stam
+ * 1999): semi-Lagrangian advection of a perturbation velocity under the
+ * Keplerian shear, curl-noise forcing with drag, Jacobi pressure projection,
+ * MacCormack density advection relaxed toward a noise-modulated profile,
+ * extruded through the vertical envelope into the context's gas texture.
+ * Fields are float[nphi][nr] (the Julia (nr, nphi) bytes); velocities are in
+ * grid cells per M-time. */
+typedef struct st_fluid st_fluid;
+enum { ST_FLUID_VU = 0, ST_FLUID_VV = 1, ST_FLUID_DENS = 2, ST_FLUID_PRS = 3, ST_FLUID_DIV = 4 };
+/* DiscFluidSim (src/disc_sim.jl:42-76): initial surface density dens0,
+ * relaxation profile base (both float[nphi][nr]), vertical envelope
+ * env float[nz][nr], shear float[nr] (phi cells per M-time); forcing, drag
+ * (per M-time), relax (per M-time).  The host computes these as the
+ * reference constructor does. */
+int st_fluid_create(st_ctx* ctx, int nr, int nphi, int nz, const float* dens0, const float* base, const float* env,
+                    const float* shear, float forcing, float drag, float relax, st_fluid** out);
+int st_fluid_destroy(st_fluid* sim);
+/* step_sim! (src/disc_sim.jl:285-321): advance dt (M-time) with `jacobi`
+ * pressure iterations (rounded up to even) and write the extruded density
+ * into the context's volume, which must have the fluid's (nr, nphi, nz).
+ * Asynchronous on the context's stream; the next trace sees the new field. */
+int st_fluid_step(st_fluid* sim, st_ctx* ctx, float dt, int jacobi);
+/* Copy one field (ST_FLUID_*) to host float[nphi][nr]. */
+int st_fluid_download(st_ctx* ctx, const st_fluid* sim, int field, float* host_out);
+/* The simulation clock (M-time). */
+int st_fluid_time(const st_fluid* sim, float* t);
 
 /* ---- batched tracing: many cameras / frames of uneven size in one launch ----
  * No counterpart in the reference (it loops _launch_trace! per frame,

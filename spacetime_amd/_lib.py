@@ -18,6 +18,7 @@ ST_OK, ST_ERR_INVALID_ARG, ST_ERR_OOM, ST_ERR_DEVICE, ST_ERR_LAUNCH, ST_ERR_UNSU
 ST_PREC_FP32, ST_PREC_MIXED, ST_PREC_FP64 = 0, 1, 2
 ST_BATCH_ONE_SHOT = -1   # st_trace_batch: one wave per wave tile instead of a persistent grid
 REC_N = 12
+ST_FLUID_VU, ST_FLUID_VV, ST_FLUID_DENS, ST_FLUID_PRS, ST_FLUID_DIV = range(5)
 ST_MAP_CHANNELS = 8       # warp-map texel: direction (3), emission (3), transmittance, escape flag
 REC_FATE, REC_STEPS, REC_DIR, REC_RGB, REC_HAM, REC_REND, REC_FLAGS, REC_RKMIN = 0, 1, 2, 5, 8, 9, 10, 11
 
@@ -101,6 +102,13 @@ SIGNATURES = {
     "st_download_stat": (C.c_int, [C.c_void_p, _F]),
     "st_trace_depth": (C.c_int, [C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs), C.c_int]),
     "st_download_depth": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_ctx_download_volume": (C.c_int, [C.c_void_p, _F, C.c_size_t]),
+    "st_fluid_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _F, _F, _F, _F, C.c_float, C.c_float,
+                                  C.c_float, C.POINTER(C.c_void_p)]),
+    "st_fluid_destroy": (C.c_int, [C.c_void_p]),
+    "st_fluid_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int]),
+    "st_fluid_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, _F]),
+    "st_fluid_time": (C.c_int, [C.c_void_p, _F]),
     "st_map_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "st_map_destroy": (C.c_int, [C.c_void_p]),
     "st_bake_warp_map": (C.c_int, [C.c_void_p, C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs)]),

@@ -175,6 +175,7 @@ class DiscVolume:
     emission_scale: float = 1.0
     opacity_scale: float = 1.0
     march_stride: int = 2
+    scale_height: float = 0.05        # h/r of the vertical envelope (the fluid sim's extrusion)
 
     def params(self) -> np.ndarray:
         nz, nphi, nr = self.density.shape
@@ -331,6 +332,14 @@ class HipPreviewContext:
         nz, nphi, nr = d.shape
         L.check(L.lib().st_ctx_set_volume(self._h, L.fptr(d), nr, nphi, nz, L.fptr(vol.params())))
         self.has_volume = True
+        self._vol_shape = (nz, nphi, nr)
+
+    def volume(self) -> np.ndarray:
+        """The context's gas texture, float32 (nz, nphi, nr)."""
+        d = self._vol_shape
+        out = np.empty(d, dtype=np.float32)
+        L.check(L.lib().st_ctx_download_volume(self._h, L.fptr(out), out.size))
+        return out
 
     def set_volume_enabled(self, enabled: bool):
         L.check(L.lib().st_ctx_set_volume_enabled(self._h, int(enabled)))

@@ -280,6 +280,15 @@ int st_ctx_set_volume_enabled(st_ctx* c, int enabled) {
     return ST_OK;
 }
 
+int st_ctx_download_volume(st_ctx* c, float* host, size_t n) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!c->has_volume) return fail(ST_ERR_INVALID_ARG, "the context has no volume");
+    if (n != (size_t)c->nr * c->nphi * c->nz) return fail(ST_ERR_INVALID_ARG, "nfloats must be nr * nphi * nz");
+    ST_HIP(hipMemcpyAsync(host, c->vol, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
 int st_ctx_set_march_stride(st_ctx* c, int stride) {
     if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
     if (!c->has_volume) return ST_OK;

@@ -1,0 +1,338 @@
+// Live fluid-disc simulation (src/disc_sim.jl:137-321): a 2D stable-fluids
+// solver on the volume's own (log s, phi) grid — semi-Lagrangian advection
+// of a perturbation velocity riding the Keplerian shear, curl-noise forcing,
+// Jacobi pressure projection, MacCormack density advection with relaxation
+// toward a noise-modulated profile — extruded through the vertical envelope
+// into the context's (nr, nphi, nz) gas texture the tracer samples.
+//
+// Fields are float[nphi][nr] (the Julia (nr, nphi) bytes); the arithmetic
+// keeps the reference's 1-based cell coordinates so backtraced positions
+// round identically.  MI355X form: the reference's ~17 dispatches per step
+// become 5 + jacobi: the divergence pass also zeroes the pressure, the
+// pressure-gradient subtraction also does the forward density advection
+// (it needs only the cell's own corrected velocity), and the backward
+// advection, the MacCormack combine and the extrusion run as one kernel (the
+// combine needs the backward result of its own cell only, and each cell
+// writes its nz-cell column).  Every kernel is one thread per cell; the
+// extrusion is the only HBM-bound part (4 B x nz written per cell).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "st_internal.h"
+#include "st_physics.h"
+
+namespace st {
+
+namespace {
+
+struct FluidGrid {
+    int nr, nphi, nz;
+};
+
+__device__ __forceinline__ int jmod_i(int a, int n) {
+    const int m = a % n;
+    return m < 0 ? m + n : m;
+}
+
+// _sim_noise (src/disc_sim.jl:94-110): value noise on the hashed lattice.
+__device__ __forceinline__ float sim_noise(float x, float y, float z) {
+    const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    float fx = x - flx, fy = y - fly, fz = z - flz;
+    fx = fx * fx * (3.0f - 2.0f * fx);
+    fy = fy * fy * (3.0f - 2.0f * fy);
+    fz = fz * fz * (3.0f - 2.0f * fz);
+    const int i1 = ix + 1, j1 = iy + 1, k1 = iz + 1;
+    const float c00 = sim_hash(ix, iy, iz) + fx * (sim_hash(i1, iy, iz) - sim_hash(ix, iy, iz));
+    const float c10 = sim_hash(ix, j1, iz) + fx * (sim_hash(i1, j1, iz) - sim_hash(ix, j1, iz));
+    const float c01 = sim_hash(ix, iy, k1) + fx * (sim_hash(i1, iy, k1) - sim_hash(ix, iy, k1));
+    const float c11 = sim_hash(ix, j1, k1) + fx * (sim_hash(i1, j1, k1) - sim_hash(ix, j1, k1));
+    const float c0 = c00 + fy * (c10 - c00);
+    const float c1 = c01 + fy * (c11 - c01);
+    return c0 + fz * (c1 - c0);
+}
+
+// Backtrace cell of a 1-based position (x, y): phi periodic, radius clamped
+// (src/disc_sim.jl:113-127).  Returns the 0-based corner indices and weights.
+struct Corners {
+    int i0, i1, j0a, j1a;   // 0-based
+    float tx, ty;
+};
+
+__device__ __forceinline__ Corners corners(float x, float y, int nr, int nphi) {
+    Corners c;
+    x = fminf(fmaxf(x, 1.0f), (float)nr);
+    int i0 = (int)floorf(x);
+    i0 = i0 < 1 ? 1 : (i0 > nr - 1 ? nr - 1 : i0);
+    c.tx = x - (float)i0;
+    const float yw = y - (float)nphi * floorf((y - 1.0f) / (float)nphi);
+    const int j0 = (int)floorf(yw);
+    c.ty = yw - (float)j0;
+    c.j0a = jmod_i(j0 - 1, nphi);
+    c.j1a = jmod_i(j0, nphi);
+    c.i0 = i0 - 1;
+    c.i1 = i0;
+    return c;
+}
+
+__device__ __forceinline__ float sample(const float* __restrict__ f, const Corners& c, int nr) {
+    const float a = f[c.j0a * nr + c.i0] + c.tx * (f[c.j0a * nr + c.i1] - f[c.j0a * nr + c.i0]);
+    const float b = f[c.j1a * nr + c.i0] + c.tx * (f[c.j1a * nr + c.i1] - f[c.j1a * nr + c.i0]);
+    return a + c.ty * (b - a);
+}
+
+// k_advect_vel! (src/disc_sim.jl:137-148)
+__global__ __launch_bounds__(256) void k_fluid_advect_vel(float* __restrict__ vu2, float* __restrict__ vv2,
+                                                          const float* __restrict__ vu,
+                                                          const float* __restrict__ vv,
+                                                          const float* __restrict__ shear, float dt, FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr, j = idx / G.nr;   // 0-based
+    const float x = (float)(i + 1) - dt * vu[idx];
+    const float y = (float)(j + 1) - dt * (vv[idx] + shear[i]);
+    const Corners c = corners(x, y, G.nr, G.nphi);
+    vu2[idx] = sample(vu, c, G.nr);
+    vv2[idx] = sample(vv, c, G.nr);
+}
+
+// k_force! (src/disc_sim.jl:151-172)
+__global__ __launch_bounds__(256) void k_fluid_force(float* __restrict__ vu, float* __restrict__ vv, float t, float dt,
+                                                     float amp, float drag, FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr + 1, j = idx / G.nr + 1;   // 1-based, as the reference
+    const float sc = 0.085f;
+    const float x = (float)i * sc, y = (float)j * sc, z = t * 0.35f;
+    const float e = 0.35f;
+    const float dpy = sim_noise(x, y + e, z) - sim_noise(x, y - e, z);
+    const float dpx = sim_noise(x + e, y, z) - sim_noise(x - e, y, z);
+    const float fu = dpy / (2.0f * e);
+    const float fv = -dpx / (2.0f * e);
+    float edge = fminf((float)i - 1.0f, (float)(G.nr - i)) * 0.125f;
+    edge = fminf(fmaxf(edge, 0.0f), 1.0f);
+    const float damp = expf(-drag * dt);
+    vu[idx] = (vu[idx] + dt * amp * fu * edge) * damp * edge;
+    vv[idx] = (vv[idx] + dt * amp * fv * edge) * damp;
+}
+
+struct Nb {
+    int ip, im, jp, jm;   // 0-based neighbour indices along r (clamped) and phi (periodic)
+};
+
+__device__ __forceinline__ Nb neighbours(int i, int j, int nr, int nphi) {
+    Nb n;
+    n.ip = i + 1 < nr ? i + 1 : nr - 1;
+    n.im = i - 1 > 0 ? i - 1 : 0;
+    n.jp = j + 1 < nphi ? j + 1 : 0;
+    n.jm = j - 1 >= 0 ? j - 1 : nphi - 1;
+    return n;
+}
+
+// k_div! (src/disc_sim.jl:174-183), plus the pressure reset of step_sim!'s
+// fill!(prs, 0) (a kernel store, not a memset, so a step can be captured)
+__global__ __launch_bounds__(256) void k_fluid_div(float* __restrict__ dv, float* __restrict__ prs,
+                                                   const float* __restrict__ vu, const float* __restrict__ vv,
+                                                   FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr, j = idx / G.nr;
+    const Nb n = neighbours(i, j, G.nr, G.nphi);
+    dv[idx] = 0.5f * (vu[j * G.nr + n.ip] - vu[j * G.nr + n.im] + vv[n.jp * G.nr + i] - vv[n.jm * G.nr + i]);
+    prs[idx] = 0.0f;
+}
+
+// k_jacobi! (src/disc_sim.jl:185-194)
+__global__ __launch_bounds__(256) void k_fluid_jacobi(float* __restrict__ p2, const float* __restrict__ p,
+                                                      const float* __restrict__ dv, FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr, j = idx / G.nr;
+    const Nb n = neighbours(i, j, G.nr, G.nphi);
+    p2[idx] = 0.25f * (p[j * G.nr + n.ip] + p[j * G.nr + n.im] + p[n.jp * G.nr + i] + p[n.jm * G.nr + i] - dv[idx]);
+}
+
+// k_subgrad! (src/disc_sim.jl:196-205) fused with the forward density
+// advection k_adv_scalar!(dens2, dens, +dt) (:208-218): the advection reads
+// only this cell's corrected velocity.
+__global__ __launch_bounds__(256) void k_fluid_subgrad_adv(float* __restrict__ vu, float* __restrict__ vv,
+                                                           const float* __restrict__ p, float* __restrict__ d1,
+                                                           const float* __restrict__ d,
+                                                           const float* __restrict__ shear, float dt, FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr, j = idx / G.nr;
+    const Nb n = neighbours(i, j, G.nr, G.nphi);
+    const float u = vu[idx] - 0.5f * (p[j * G.nr + n.ip] - p[j * G.nr + n.im]);
+    const float v = vv[idx] - 0.5f * (p[n.jp * G.nr + i] - p[n.jm * G.nr + i]);
+    vu[idx] = u;
+    vv[idx] = v;
+    const float x = (float)(i + 1) - dt * u;
+    const float y = (float)(j + 1) - dt * (v + shear[i]);
+    d1[idx] = sample(d, corners(x, y, G.nr, G.nphi), G.nr);
+}
+
+// The backward re-advection k_adv_scalar!(dens3, dens2, -dt), the MacCormack
+// combine k_maccormack! (src/disc_sim.jl:228-257) and the extrusion
+// k_extrude! (:260-270) of the new density, per cell.
+__global__ __launch_bounds__(256) void k_fluid_mac_extrude(float* __restrict__ dnew, const float* __restrict__ d,
+                                                           const float* __restrict__ d1,
+                                                           const float* __restrict__ vu,
+                                                           const float* __restrict__ vv,
+                                                           const float* __restrict__ shear,
+                                                           const float* __restrict__ base,
+                                                           const float* __restrict__ env, float* __restrict__ vol3,
+                                                           float t, float dt, float relax, FluidGrid G) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.nr * G.nphi) return;
+    const int i = idx % G.nr, j = idx / G.nr;
+    const float u = vu[idx], v = vv[idx] + shear[i];
+    // backward: d2 = d1 sampled at the forward-displaced point
+    const float d2 = sample(d1, corners((float)(i + 1) + dt * u, (float)(j + 1) + dt * v, G.nr, G.nphi), G.nr);
+    // limiter: the bilinear corners of d at the backtraced point
+    const Corners c = corners((float)(i + 1) - dt * u, (float)(j + 1) - dt * v, G.nr, G.nphi);
+    const float a00 = d[c.j0a * G.nr + c.i0], a10 = d[c.j0a * G.nr + c.i1];
+    const float a01 = d[c.j1a * G.nr + c.i0], a11 = d[c.j1a * G.nr + c.i1];
+    const float lo = fminf(fminf(a00, a10), fminf(a01, a11));
+    const float hi = fmaxf(fmaxf(a00, a10), fmaxf(a01, a11));
+    const float a = fminf(fmaxf(d1[idx] + 0.5f * (d[idx] - d2), lo), hi);
+    const float n = sim_noise((float)(i + 1) * 0.11f, (float)(j + 1) * 0.11f, t * 0.25f);
+    const float tgt = base[idx] * expf(1.2f * (2.0f * n - 1.0f));
+    const float dn = fmaxf(a + relax * dt * (tgt - a), 0.0f);
+    dnew[idx] = dn;
+    const size_t plane = (size_t)G.nr * G.nphi;
+    for (int k = 0; k < G.nz; ++k) vol3[(size_t)k * plane + idx] = dn * env[k * G.nr + i];
+}
+
+}  // namespace
+
+}  // namespace st
+
+// ---- C ABI ----
+using st::fail;
+
+struct st_fluid {
+    int device = 0;
+    int nr = 0, nphi = 0, nz = 0;
+    float *vu = nullptr, *vv = nullptr, *vu2 = nullptr, *vv2 = nullptr;
+    float *dens = nullptr, *dens2 = nullptr, *dens3 = nullptr, *base = nullptr;
+    float *prs = nullptr, *prs2 = nullptr, *div = nullptr;
+    float *shear = nullptr, *env = nullptr;
+    float t = 0.0f, forcing = 6.0f, drag = 0.15f, relax = 0.05f;
+};
+
+namespace {
+
+void fluid_free(st_fluid* s) {
+    float** bufs[] = {&s->vu, &s->vu2, &s->vv, &s->vv2, &s->dens, &s->dens2, &s->dens3, &s->base,
+                      &s->prs, &s->prs2, &s->div, &s->shear, &s->env};
+    for (float** b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int st_fluid_create(st_ctx* c, int nr, int nphi, int nz, const float* dens0, const float* base, const float* env,
+                    const float* shear, float forcing, float drag, float relax, st_fluid** out) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!c || !dens0 || !base || !env || !shear) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (nr < 3 || nphi < 3 || nz < 1) return fail(ST_ERR_INVALID_ARG, "the fluid grid must be at least 3x3x1");
+    if ((size_t)nr * nphi * nz >= ((size_t)1 << 30)) return fail(ST_ERR_INVALID_ARG, "fluid grid too large");
+    ST_HIP(hipSetDevice(c->device));
+    st_fluid* s = new st_fluid();
+    s->device = c->device;
+    s->nr = nr; s->nphi = nphi; s->nz = nz;
+    s->forcing = forcing; s->drag = drag; s->relax = relax;
+    const size_t n = (size_t)nr * nphi;
+    float** planes[] = {&s->vu, &s->vu2, &s->vv, &s->vv2, &s->dens, &s->dens2, &s->dens3, &s->base,
+                        &s->prs, &s->prs2, &s->div};
+    for (float** b : planes) {
+        if (hipMalloc(b, sizeof(float) * n) != hipSuccess || hipMemset(*b, 0, sizeof(float) * n) != hipSuccess) {
+            fluid_free(s);
+            delete s;
+            return fail(ST_ERR_OOM, "fluid buffer allocation failed");
+        }
+    }
+    if (hipMalloc(&s->shear, sizeof(float) * nr) != hipSuccess ||
+        hipMalloc(&s->env, sizeof(float) * (size_t)nr * nz) != hipSuccess ||
+        hipMemcpy(s->dens, dens0, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->base, base, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->env, env, sizeof(float) * (size_t)nr * nz, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->shear, shear, sizeof(float) * nr, hipMemcpyHostToDevice) != hipSuccess) {
+        fluid_free(s);
+        delete s;
+        return fail(ST_ERR_DEVICE, "fluid buffer upload failed");
+    }
+    *out = s;
+    return ST_OK;
+}
+
+int st_fluid_destroy(st_fluid* s) {
+    if (!s) return ST_OK;
+    (void)hipSetDevice(s->device);
+    fluid_free(s);
+    delete s;
+    return ST_OK;
+}
+
+int st_fluid_step(st_fluid* s, st_ctx* c, float dt, int jacobi) {
+    if (!s || !c) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (s->device != c->device) return fail(ST_ERR_INVALID_ARG, "fluid and context are on different devices");
+    if (!c->has_volume || c->nr != s->nr || c->nphi != s->nphi || c->nz != s->nz)
+        return fail(ST_ERR_INVALID_ARG, "the context needs a volume of the fluid's (nr, nphi, nz)");
+    if (jacobi < 0 || jacobi > 10000) return fail(ST_ERR_INVALID_ARG, "jacobi must be in [0, 10000]");
+    const st::FluidGrid G{s->nr, s->nphi, s->nz};
+    const int n = s->nr * s->nphi;
+    const dim3 grid((n + 255) / 256), blk(256);
+    hipStream_t q = c->stream;
+    ST_HIP(hipEventRecord(c->ev[0], q));
+    hipLaunchKernelGGL(st::k_fluid_advect_vel, grid, blk, 0, q, s->vu2, s->vv2, s->vu, s->vv, s->shear, dt, G);
+    std::swap(s->vu, s->vu2);
+    std::swap(s->vv, s->vv2);
+    hipLaunchKernelGGL(st::k_fluid_force, grid, blk, 0, q, s->vu, s->vv, s->t, dt, s->forcing, s->drag, G);
+    hipLaunchKernelGGL(st::k_fluid_div, grid, blk, 0, q, s->div, s->prs, s->vu, s->vv, G);
+    for (int it = 0; it < (jacobi + 1) / 2; ++it) {
+        hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs2, s->prs, s->div, G);
+        hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs, s->prs2, s->div, G);
+    }
+    hipLaunchKernelGGL(st::k_fluid_subgrad_adv, grid, blk, 0, q, s->vu, s->vv, s->prs, s->dens2, s->dens, s->shear,
+                       dt, G);
+    hipLaunchKernelGGL(st::k_fluid_mac_extrude, grid, blk, 0, q, s->dens3, s->dens, s->dens2, s->vu, s->vv,
+                       s->shear, s->base, s->env, c->vol, s->t, dt, s->relax, G);
+    ST_HIP(hipGetLastError());
+    std::swap(s->dens, s->dens3);
+    ST_HIP(hipEventRecord(c->ev[1], q));
+    ST_HIP(hipEventRecord(c->ev[2], q));
+    s->t += dt;
+    return ST_OK;
+}
+
+int st_fluid_download(st_ctx* c, const st_fluid* s, int field, float* host) {
+    if (!c || !s || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    const float* src = nullptr;
+    switch (field) {
+        case ST_FLUID_VU: src = s->vu; break;
+        case ST_FLUID_VV: src = s->vv; break;
+        case ST_FLUID_DENS: src = s->dens; break;
+        case ST_FLUID_PRS: src = s->prs; break;
+        case ST_FLUID_DIV: src = s->div; break;
+        default: return fail(ST_ERR_INVALID_ARG, "unknown fluid field");
+    }
+    ST_HIP(hipMemcpyAsync(host, src, sizeof(float) * (size_t)s->nr * s->nphi, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_fluid_time(const st_fluid* s, float* t) {
+    if (!s || !t) return fail(ST_ERR_INVALID_ARG, "null argument");
+    *t = s->t;
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -150,7 +150,8 @@ def disc_volume(seed: int = 55, nr: int = 512, nphi: int = 512, nz: int = 32, h_
     hs = h_over_r * np.exp(ls)                       # scale height per radius
     prof = np.exp(-0.5 * (z[:, None] / hs[None, :]) ** 2)   # (nz, nr)
     dens = (turb * prof[:, None, :]).astype(np.float32)
-    return api.DiscVolume(dens, float(np.log(s_in)), float(np.log(s_out)), float(z_max), emission, opacity, 2)
+    return api.DiscVolume(dens, float(np.log(s_in)), float(np.log(s_out)), float(z_max), emission, opacity, 2,
+                          h_over_r)
 
 
 def orbit_cameras(n_frames: int = 120, r: float = 15.0, incl_deg: float = 80.0, fov: float = 0.75):
