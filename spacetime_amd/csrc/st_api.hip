@@ -378,6 +378,7 @@ int st_trace_depth(st_ctx* c, const float cam[49], const double* cam64, const fl
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = false;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 
@@ -480,6 +481,7 @@ int st_trace(st_ctx* c, const float cam[49], const double* cam64, const float sp
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = a->precision == ST_PREC_MIXED;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 
@@ -511,10 +513,12 @@ int st_get_stats(st_ctx* c, st_stats* s) {
     std::memset(s, 0, sizeof(*s));
     if (!c->have_last) return ST_OK;
     ST_HIP(hipStreamSynchronize(c->stream));
-    unsigned long long cnt[st::kNumCounters];
+    unsigned long long cnt[st::kNumCounters] = {0, 0, 0, 0, 0};
     unsigned q2[2] = {0u, 0u};
-    ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    ST_HIP(hipMemcpy(q2, c->qcount, sizeof(q2), hipMemcpyDeviceToHost));
+    if (c->last_counters) {
+        ST_HIP(hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        ST_HIP(hipMemcpy(q2, c->qcount, sizeof(q2), hipMemcpyDeviceToHost));
+    }
     const unsigned q = q2[0] + q2[1];
     s->rays = cnt[3];
     s->steps_fp32 = cnt[0] + cnt[2];
@@ -691,6 +695,7 @@ int st_trace_batch(st_ctx* c, const st_trace_args* a, int tile, int waves_per_cu
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = a->precision == ST_PREC_MIXED;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 
@@ -730,6 +735,7 @@ int st_trace_batch_graph(st_ctx* c, const st_trace_args* a, int tile, int waves_
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = a->precision == ST_PREC_MIXED;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 

@@ -82,39 +82,33 @@ __device__ __forceinline__ float sample(const float* __restrict__ f, const Corne
     return a + c.ty * (b - a);
 }
 
-// k_advect_vel! (src/disc_sim.jl:137-148)
-__global__ __launch_bounds__(256) void k_fluid_advect_vel(float* __restrict__ vu2, float* __restrict__ vv2,
-                                                          const float* __restrict__ vu,
-                                                          const float* __restrict__ vv,
-                                                          const float* __restrict__ shear, float dt, FluidGrid G) {
+// k_advect_vel! followed by k_force! (src/disc_sim.jl:137-172): the forcing
+// reads only the cell's own advected velocity, so one thread does both.
+__global__ __launch_bounds__(256) void k_fluid_advect_force(float* __restrict__ vu2, float* __restrict__ vv2,
+                                                            const float* __restrict__ vu,
+                                                            const float* __restrict__ vv,
+                                                            const float* __restrict__ shear, float t, float dt,
+                                                            float amp, float drag, FluidGrid G) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= G.nr * G.nphi) return;
     const int i = idx % G.nr, j = idx / G.nr;   // 0-based
     const float x = (float)(i + 1) - dt * vu[idx];
     const float y = (float)(j + 1) - dt * (vv[idx] + shear[i]);
     const Corners c = corners(x, y, G.nr, G.nphi);
-    vu2[idx] = sample(vu, c, G.nr);
-    vv2[idx] = sample(vv, c, G.nr);
-}
-
-// k_force! (src/disc_sim.jl:151-172)
-__global__ __launch_bounds__(256) void k_fluid_force(float* __restrict__ vu, float* __restrict__ vv, float t, float dt,
-                                                     float amp, float drag, FluidGrid G) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= G.nr * G.nphi) return;
-    const int i = idx % G.nr + 1, j = idx / G.nr + 1;   // 1-based, as the reference
+    const float u = sample(vu, c, G.nr), v = sample(vv, c, G.nr);
+    const int i1 = i + 1, j1 = j + 1;           // the forcing's 1-based coordinates
     const float sc = 0.085f;
-    const float x = (float)i * sc, y = (float)j * sc, z = t * 0.35f;
+    const float fx = (float)i1 * sc, fy = (float)j1 * sc, fz = t * 0.35f;
     const float e = 0.35f;
-    const float dpy = sim_noise(x, y + e, z) - sim_noise(x, y - e, z);
-    const float dpx = sim_noise(x + e, y, z) - sim_noise(x - e, y, z);
+    const float dpy = sim_noise(fx, fy + e, fz) - sim_noise(fx, fy - e, fz);
+    const float dpx = sim_noise(fx + e, fy, fz) - sim_noise(fx - e, fy, fz);
     const float fu = dpy / (2.0f * e);
     const float fv = -dpx / (2.0f * e);
-    float edge = fminf((float)i - 1.0f, (float)(G.nr - i)) * 0.125f;
+    float edge = fminf((float)i1 - 1.0f, (float)(G.nr - i1)) * 0.125f;
     edge = fminf(fmaxf(edge, 0.0f), 1.0f);
     const float damp = expf(-drag * dt);
-    vu[idx] = (vu[idx] + dt * amp * fu * edge) * damp * edge;
-    vv[idx] = (vv[idx] + dt * amp * fv * edge) * damp;
+    vu2[idx] = (u + dt * amp * fu * edge) * damp * edge;
+    vv2[idx] = (v + dt * amp * fv * edge) * damp;
 }
 
 struct Nb {
@@ -153,6 +147,85 @@ __global__ __launch_bounds__(256) void k_fluid_jacobi(float* __restrict__ p2, co
     p2[idx] = 0.25f * (p[j * G.nr + n.ip] + p[j * G.nr + n.im] + p[n.jp * G.nr + i] + p[n.jm * G.nr + i] - dv[idx]);
 }
 
+// k_div!, fill!(prs, 0) and J sweeps of k_jacobi! (src/disc_sim.jl:174-194,
+// 300-304) in one launch: a block loads its 32 x 16 tile of the velocity
+// with a halo of J + 1 cells into LDS, forms the divergence on the tile
+// +- J, and runs the J sweeps in LDS on a region that shrinks by one cell
+// per sweep (sweep s needs sweep s - 1 only one cell further out), writing
+// the tile's pressure and divergence.  The arithmetic is the unfused
+// kernels' to the bit; the halo is recomputed by neighbouring blocks.  The
+// radial neighbour of an edge cell is the cell itself (the reference's
+// clamp); phi wraps.
+constexpr int kTileR = 32, kTilePhi = 16, kMaxFusedJacobi = 16;
+
+__global__ __launch_bounds__(1024) void k_fluid_div_jacobi(float* __restrict__ prs, float* __restrict__ dv_out,
+                                                          const float* __restrict__ vu,
+                                                          const float* __restrict__ vv, int J, FluidGrid G) {
+    extern __shared__ float lds[];
+    const int H = J + 1;
+    const int LX = kTileR + 2 * H, LY = kTilePhi + 2 * H, LN = LX * LY;
+    float* dv = lds;
+    float* pa = lds + LN;
+    float* pb = lds + 2 * LN;
+    const int gi0 = blockIdx.x * kTileR - H;      // global radial index of LDS column 0
+    const int gj0 = blockIdx.y * kTilePhi - H;    // global phi index of LDS row 0 (wraps)
+    // 1024 threads as 64 columns x 16 rows striding over the LDS tile (no
+    // integer division in the loops; few serial rows per sweep)
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // LDS column of the (clamped) global radial index gi; valid for the
+    // columns a real cell's neighbours can reach
+    auto col = [&](int gi) { return (gi < 0 ? 0 : (gi > G.nr - 1 ? G.nr - 1 : gi)) - gi0; };
+    for (int ly = ty; ly < LY; ly += 16) {
+        const int gj = jmod_i(gj0 + ly, G.nphi);
+        for (int lx = tx; lx < LX; lx += 64) {
+            const int gi = gi0 + lx;
+            const bool in = gi >= 0 && gi < G.nr;
+            pa[ly * LX + lx] = in ? vu[gj * G.nr + gi] : 0.0f;
+            pb[ly * LX + lx] = in ? vv[gj * G.nr + gi] : 0.0f;
+        }
+    }
+    __syncthreads();
+    // divergence on the tile +- J (columns/rows 1 .. L - 2), real cells only
+    for (int ly = ty; ly < LY; ly += 16)
+        for (int lx = tx; lx < LX; lx += 64) {
+            const int gi = gi0 + lx;
+            float d = 0.0f;
+            if (lx >= 1 && lx < LX - 1 && ly >= 1 && ly < LY - 1 && gi >= 0 && gi < G.nr)
+                d = 0.5f * (pa[ly * LX + col(gi + 1)] - pa[ly * LX + col(gi - 1)] + pb[(ly + 1) * LX + lx] -
+                            pb[(ly - 1) * LX + lx]);
+            dv[ly * LX + lx] = d;
+        }
+    __syncthreads();
+    for (int ly = ty; ly < LY; ly += 16)
+        for (int lx = tx; lx < LX; lx += 64) pa[ly * LX + lx] = 0.0f;   // pressure starts at zero
+    __syncthreads();
+    float* src = pa;
+    float* dst = pb;
+    for (int sw = 1; sw <= J; ++sw) {
+        const int m = 1 + sw;                    // margin: the region is the tile +- (J - sw)
+        for (int ly = m + ty; ly < LY - m; ly += 16)
+            for (int lx = m + tx; lx < LX - m; lx += 64) {
+                const int gi = gi0 + lx;
+                if (gi < 0 || gi >= G.nr) continue;
+                const int e = ly * LX + lx;
+                dst[e] = 0.25f * (src[ly * LX + col(gi + 1)] + src[ly * LX + col(gi - 1)] + src[e + LX] +
+                                  src[e - LX] - dv[e]);
+            }
+        __syncthreads();
+        float* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    for (int y = ty; y < kTilePhi; y += 16) {
+        const int gj = blockIdx.y * kTilePhi + y;
+        const int gi = blockIdx.x * kTileR + tx;
+        if (tx >= kTileR || gi >= G.nr || gj >= G.nphi) continue;
+        const int l = (y + H) * LX + (tx + H);
+        prs[gj * G.nr + gi] = src[l];
+        dv_out[gj * G.nr + gi] = dv[l];
+    }
+}
+
 // k_subgrad! (src/disc_sim.jl:196-205) fused with the forward density
 // advection k_adv_scalar!(dens2, dens, +dt) (:208-218): the advection reads
 // only this cell's corrected velocity.
@@ -184,8 +257,11 @@ __global__ __launch_bounds__(256) void k_fluid_mac_extrude(float* __restrict__ d
                                                            const float* __restrict__ base,
                                                            const float* __restrict__ env, float* __restrict__ vol3,
                                                            float t, float dt, float relax, FluidGrid G) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= G.nr * G.nphi) return;
+    const int idx0 = blockIdx.x * 256 + threadIdx.x;
+    const int n = G.nr * G.nphi;
+    // lanes past the end compute a harmless copy of the last cell so that the
+    // whole wave reaches the quad exchange below
+    const int idx = idx0 < n ? idx0 : n - 1;
     const int i = idx % G.nr, j = idx / G.nr;
     const float u = vu[idx], v = vv[idx] + shear[i];
     // backward: d2 = d1 sampled at the forward-displaced point
@@ -197,12 +273,29 @@ __global__ __launch_bounds__(256) void k_fluid_mac_extrude(float* __restrict__ d
     const float lo = fminf(fminf(a00, a10), fminf(a01, a11));
     const float hi = fmaxf(fmaxf(a00, a10), fmaxf(a01, a11));
     const float a = fminf(fmaxf(d1[idx] + 0.5f * (d[idx] - d2), lo), hi);
-    const float n = sim_noise((float)(i + 1) * 0.11f, (float)(j + 1) * 0.11f, t * 0.25f);
-    const float tgt = base[idx] * expf(1.2f * (2.0f * n - 1.0f));
+    const float noise = sim_noise((float)(i + 1) * 0.11f, (float)(j + 1) * 0.11f, t * 0.25f);
+    const float tgt = base[idx] * expf(1.2f * (2.0f * noise - 1.0f));
     const float dn = fmaxf(a + relax * dt * (tgt - a), 0.0f);
-    dnew[idx] = dn;
-    const size_t plane = (size_t)G.nr * G.nphi;
-    for (int k = 0; k < G.nz; ++k) vol3[(size_t)k * plane + idx] = dn * env[k * G.nr + i];
+    if (idx0 < n) dnew[idx0] = dn;
+    const size_t plane = (size_t)n;
+    if ((G.nr & 3) == 0) {
+        // 4 consecutive cells of one radial row per lane quad: each lane
+        // writes every 4th z-plane of the quad as one 16-byte store
+        const int lane = threadIdx.x & 63, q = lane & 3, qb = lane & ~3;
+        const float d0 = __shfl(dn, qb, 64), d1q = __shfl(dn, qb + 1, 64);
+        const float d2q = __shfl(dn, qb + 2, 64), d3q = __shfl(dn, qb + 3, 64);
+        const int base = idx0 - q;                 // multiple of 4, in one radial row
+        if (base < n) {
+            const int ib = base % G.nr;
+            for (int k = q; k < G.nz; k += 4) {
+                const float4 e = *reinterpret_cast<const float4*>(env + (size_t)k * G.nr + ib);
+                *reinterpret_cast<float4*>(vol3 + (size_t)k * plane + base) =
+                    make_float4(d0 * e.x, d1q * e.y, d2q * e.z, d3q * e.w);
+            }
+        }
+    } else if (idx0 < n) {
+        for (int k = 0; k < G.nz; ++k) vol3[(size_t)k * plane + idx0] = dn * env[k * G.nr + i];
+    }
 }
 
 }  // namespace
@@ -292,14 +385,23 @@ int st_fluid_step(st_fluid* s, st_ctx* c, float dt, int jacobi) {
     const dim3 grid((n + 255) / 256), blk(256);
     hipStream_t q = c->stream;
     ST_HIP(hipEventRecord(c->ev[0], q));
-    hipLaunchKernelGGL(st::k_fluid_advect_vel, grid, blk, 0, q, s->vu2, s->vv2, s->vu, s->vv, s->shear, dt, G);
+    hipLaunchKernelGGL(st::k_fluid_advect_force, grid, blk, 0, q, s->vu2, s->vv2, s->vu, s->vv, s->shear, s->t, dt,
+                       s->forcing, s->drag, G);
     std::swap(s->vu, s->vu2);
     std::swap(s->vv, s->vv2);
-    hipLaunchKernelGGL(st::k_fluid_force, grid, blk, 0, q, s->vu, s->vv, s->t, dt, s->forcing, s->drag, G);
-    hipLaunchKernelGGL(st::k_fluid_div, grid, blk, 0, q, s->div, s->prs, s->vu, s->vv, G);
-    for (int it = 0; it < (jacobi + 1) / 2; ++it) {
-        hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs2, s->prs, s->div, G);
-        hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs, s->prs2, s->div, G);
+    const int sweeps = 2 * ((jacobi + 1) / 2);
+    if (sweeps <= st::kMaxFusedJacobi) {
+        const int H = sweeps + 1;
+        const size_t lds = sizeof(float) * 3 * (size_t)(st::kTileR + 2 * H) * (st::kTilePhi + 2 * H);
+        const dim3 tiles((s->nr + st::kTileR - 1) / st::kTileR, (s->nphi + st::kTilePhi - 1) / st::kTilePhi);
+        hipLaunchKernelGGL(st::k_fluid_div_jacobi, tiles, dim3(1024), lds, q, s->prs, s->div, s->vu, s->vv, sweeps,
+                           G);
+    } else {
+        hipLaunchKernelGGL(st::k_fluid_div, grid, blk, 0, q, s->div, s->prs, s->vu, s->vv, G);
+        for (int it = 0; it < sweeps / 2; ++it) {
+            hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs2, s->prs, s->div, G);
+            hipLaunchKernelGGL(st::k_fluid_jacobi, grid, blk, 0, q, s->prs, s->prs2, s->div, G);
+        }
     }
     hipLaunchKernelGGL(st::k_fluid_subgrad_adv, grid, blk, 0, q, s->vu, s->vv, s->prs, s->dens2, s->dens, s->shear,
                        dt, G);
@@ -309,6 +411,9 @@ int st_fluid_step(st_fluid* s, st_ctx* c, float dt, int jacobi) {
     std::swap(s->dens, s->dens3);
     ST_HIP(hipEventRecord(c->ev[1], q));
     ST_HIP(hipEventRecord(c->ev[2], q));
+    c->last_mixed = false;
+    c->have_last = true;
+    c->last_counters = false;   // a fluid step counts no rays
     s->t += dt;
     return ST_OK;
 }

@@ -102,6 +102,7 @@ struct st_ctx {
     hipEvent_t ev_band = nullptr, ev_join = nullptr;
     bool last_mixed = false;
     bool have_last = false;
+    bool last_counters = true;   // false after a call that counts nothing (a fluid step)
     // batch
     int nframes = 0;
     std::vector<st::FrameRef> frames_host;

@@ -470,6 +470,7 @@ int st_mlp_render(st_mlp* m, st_ctx* c, const st_surrogate_view* v, const float 
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = true;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 

@@ -225,6 +225,7 @@ int st_bake_warp_map(st_ctx* c, st_map* m, const float cam[49], const double* ca
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = false;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 
@@ -269,6 +270,7 @@ int st_warp_sample(st_ctx* c, const st_map* m0, const st_map* m1, const float pa
     ST_HIP(hipEventRecord(c->ev[2], c->stream));
     c->last_mixed = false;
     c->have_last = true;
+    c->last_counters = true;
     return ST_OK;
 }
 
