@@ -8,6 +8,7 @@ C ABI, camera/sky/LUT already resident in HBM.  Prints one JSON line.
     python bench.py --workload volume4k    # 4K Kerr a=0.99 gas disc, 8 frames/launch
     python bench.py --workload warp1080p   # 1080p frames composited from baked warp maps
     python bench.py --workload fluid       # live fluid-disc step on the 512x512x32 volume grid
+    python bench.py --workload layered1080p  # layered engine: sky fan + half-res disc layer + composite
 
 The default line also carries, measured in the same run: the mixed
 FP32/FP64 policy, the surrogate + RK4-fallback form (when trained weights
@@ -445,6 +446,138 @@ def bench_fluid(args):
     return line
 
 
+# ---------------------------------------------------------------------------
+# layered real-time engine at 1080p
+# ---------------------------------------------------------------------------
+def bench_layered(args):
+    from spacetime_amd import api, layered, scenes
+
+    W, H = args.width, args.height
+    st = api.Schwarzschild(1.0)
+    disc = scenes.config_disc()
+    ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=disc)     # the preview context: dt 0.1
+    star_params = ctx.set_starfield(height=H)
+    cam = api.Camera((0.0, -30.0, 3.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0))
+    gate = 1.1 * disc.outer_radius
+    sky = layered.SkyFanState(ctx)
+    lw, lh = W // 2, H // 2
+    layer = layered.Image(ctx, lw, lh)
+
+    # a camera falling inward 0.002M a frame: the fans are integrated every frame
+    frame = [0]
+
+    def step():
+        frame[0] += 1
+        c = api.Camera((0.0, -30.0 + 0.002 * frame[0], 3.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0))
+        layered.update_sky_fan(sky, ctx, c.pos, st, gate=gate)
+        layered.render_layered(ctx, sky, c, st, layer)
+
+    wall = timed(step, ctx.sync, args.steps, args.warmup)
+
+    def step_orbit():
+        layered.update_sky_fan(sky, ctx, cam.pos, st, gate=gate)
+        layered.render_layered(ctx, sky, cam, st, layer)
+
+    wall_reuse = timed(step_orbit, ctx.sync, args.steps, args.warmup)
+    fan_ms, lay_ms, comp_ms = [], [], []
+    for k in range(6):
+        layered.update_sky_fan(sky, ctx, cam.pos + np.array([0.0, 1e-3 * (k + 1), 0.0]), st, gate=gate)
+        fan_ms.append(ctx.stats()["ms_total"])
+        layered.layer_pass(ctx, sky, cam, st, layer)
+        s = ctx.stats()
+        lay_ms.append(s["ms_total"])
+        layered.composite(ctx, sky, cam, st, layer)
+        comp_ms.append(ctx.stats()["ms_total"])
+    fms, lms, cms = (float(np.median(x[1:])) for x in (fan_ms, lay_ms, comp_ms))
+    # the live alternative: a direct RK4 trace of the same camera's full frame
+    rule = api.StepRule(dt=ctx.dt, r_escape_factor=ctx.r_escape_factor)
+    r_esc = rule.escape_radius(cam, 1.0)
+    rule.nmax = layered.layered_nmax(ctx, r_esc, 1.0, ctx.dt)
+    c32, c64 = api.ks_cam_params(cam, 1.0, 0.0)
+    sp = api.spacetime_params(st, cam, rule)
+    live = []
+    for _ in range(4):
+        ctx.clear()
+        ctx.trace(c32, c64, sp, api.trace_args(W, H, rule))
+        live.append(ctx.stats()["ms_total"])
+    live_ms = float(np.median(live[1:]))
+    line = header(args, W, H, W * H, wall, f"Schwarzschild {W}x{H} layered frame, camera falling from r=30M "
+                  f"(the fans re-integrated every frame): 4096+1024-entry sky fan, {lw}x{lh} thin-disc layer "
+                  f"gate-culled at {gate:g}M, composite with the procedural starfield", "f32")
+    line["metric"] = "frames/s, layered engine at 1080p (sky fan + half-res disc layer + composite)"
+    line["unit"] = "frames/s"
+    line["value"] = round(1.0 / wall, 1)
+    del line["config"]["rays_per_step"]
+    line["config"]["pixels_per_step"] = W * H
+    # dominant kernel: the layer trace (FP32 VALU, the tracer's step arithmetic)
+    rays, steps = int(s["rays"]), int(s["steps_fp32"])
+    line["roofline"] = roofline(steps, rays, lms, F_STEP_SCHW, F_RAY_SCHW)
+    line["roofline"]["note"] = ("the layer trace: every launched ray pays ray generation and the fan gate test; "
+                                "culled rays take no step")
+    line["stages_ms"] = {"sky_fan": round(fms, 4), "layer": round(lms, 4), "composite": round(cms, 4),
+                         "layer_rays": rays, "layer_steps": steps}
+    line["fixed_radius"] = {"frames_per_s": round(1.0 / wall_reuse, 1),
+                            "note": "camera at a constant radius (an orbit): the fans are reused, identical results"}
+    line["live_trace"] = {"kernel_ms": round(live_ms, 4),
+                          "speedup_of_layered": round(live_ms / (fms + lms + cms), 2),
+                          "note": "plain FP32 RK4 trace of the same camera's full frame, same rule and scene"}
+    line["cpu_baseline"] = None if args.no_cpu else layered_cpu_baseline(ctx, cam, W, H, lw, lh, gate, disc,
+                                                                          star_params)
+    for o in (layer, sky):
+        o.close()
+    ctx.close()
+    return line
+
+
+def layered_cpu_baseline(ctx, cam, W, H, lw, lh, gate, disc, sp):
+    """The oracle's FP64 layered frame on one core: the fan in full, the layer
+    and the composite on every 4th row, scaled by 4."""
+    try:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle
+        from spacetime_amd import api, layered, scenes
+    except Exception as e:  # the checker is absent: report it, do not guess
+        return {"value": None, "unit": "frames/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    r = float(np.linalg.norm(cam.pos))
+    r_esc = ctx.r_escape_factor * max(r, 15.0)
+    nmax = layered.layered_nmax(ctx, r_esc, 1.0, ctx.dt)
+    cam20 = layered.fan_camera_block(r, 1.0).astype(np.float64)
+    skp = layered.sky_params_block(cam.pos, 1.0, r_esc, gate).astype(np.float64)
+    t0 = time.perf_counter()
+    fan, _ = oracle.sky_fan(cam20, skp, nmax, ctx.dt, 4096)
+    skp = oracle.fan_band(fan, skp, 3)
+    fine, _ = oracle.sky_fan(cam20, skp, nmax, ctx.dt, 1024, band=True)
+    t_fan = time.perf_counter() - t0
+    bb = disc.blackbody
+    dd = dict(inner=disc.inner_radius, outer=disc.outer_radius, falloff=disc.density_falloff, tmin=bb.table_min,
+              tmax=bb.table_max, lut=oracle.blackbody_lut(bb.wb_temperature, bb.table_min, bb.table_max,
+                                                          bb.table_size), T0=disc.T0, r0=disc.r0, hold=disc.hold_isco)
+    c32, c64 = api.ks_cam_params(cam, 1.0, 0.0)
+    kw = dict(M=1.0, a=0.0, cam_pos=cam.pos, cam_target=cam.target, cam_up=cam.up, fov=cam.fov_factor,
+              r_escape=float(np.float32(r_esc)), dt=ctx.dt, h_coef=0.16, h_floor=1.0, h_cap=8.0, kerr_shrink=True,
+              capture_mode=0, nmax=nmax, esc_interp=False, bardeen=True, cam=c64)
+    lay_sc = oracle.Scene(width=lw, height=lh, disc=dd, **kw)
+    stride = 4          # every 4th row of the layer and of the frame
+    layer = np.zeros((lh, lw, 4), np.float32)
+    layer[..., 3] = 1.0
+    t0 = time.perf_counter()
+    for j in range(0, lh, stride):
+        for i in range(lw):
+            layer[j, i] = lay_sc.trace_layer(i, j, fan, skp)
+    t_layer = (time.perf_counter() - t0) * stride
+    slut = oracle.blackbody_lut(10000.0, float(sp[11]), float(sp[12]), int(sp[13]))
+    comp_sc = oracle.Scene(width=W, height=H, sky=scenes.sky(), stars=(sp, slut), **kw)
+    t0 = time.perf_counter()
+    for j in range(0, H, stride):
+        comp_sc.sky_composite(fan, fine, skp, layer, row0=j, rows=1)
+    t_comp = (time.perf_counter() - t0) * stride
+    frame = t_fan + t_layer + t_comp
+    return {"value": round(1.0 / frame, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle on 1 thread: 4096+1024 fan ({t_fan:.2f}s), every {stride}th row of the "
+                      f"{lw}x{lh} layer ({t_layer / stride:.1f}s) and of the {W}x{H} composite "
+                      f"({t_comp / stride:.1f}s), scaled to the frame ({frame:.1f}s/frame)"}
+
+
 def cpu_baseline(sc):
     """The FP64 oracle (a C++ restatement of the reference CPU path) timed on
     this host: fixed-step RK4 (the same algorithm) over the full frame with
@@ -494,7 +627,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="kerr09_1080p", choices=["kerr09_1080p", "batch64", "volume4k", "warp1080p",
-                                                                 "fluid"])
+                                                                 "fluid", "layered1080p"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the mixed / surrogate / D2H legs")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
@@ -513,6 +646,8 @@ def main():
         line = bench_warp(args)
     elif args.workload == "fluid":
         line = bench_fluid(args)
+    elif args.workload == "layered1080p":
+        line = bench_layered(args)
     else:
         line = bench_frame(args)
     print(json.dumps(line), flush=True)

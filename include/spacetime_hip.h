@@ -183,7 +183,9 @@ int st_trace_depth(st_ctx* ctx, const float cam_params[49], const double* cam_pa
 int st_download_depth(st_ctx* ctx, float* host_out, size_t nfloats);
 /* Copy the per-pixel records to host float[H][W][ST_REC_N]. */
 int st_download_records(st_ctx* ctx, float* host_rec);
-/* Counters and device timings of the last trace call (synchronises). */
+/* Counters and device timings of the last trace call (synchronises).  A sky
+ * fan update or composite replaces the timings and keeps the counters of the
+ * trace before it. */
 int st_get_stats(st_ctx* ctx, st_stats* out);
 
 /* ---- baked warp maps (src/metal.jl:3173-3372) ----
@@ -216,6 +218,75 @@ int st_map_upload(st_ctx* ctx, st_map* map, const float* host, size_t nfloats);
  * context's sky and starfield are sampled against the blended direction.
  * Asynchronous on the context's stream. */
 int st_warp_sample(st_ctx* ctx, const st_map* map0, const st_map* map1, const float params[24]);
+
+/* ---- layered engine (src/metal.jl:2393-3142) ----
+ * By spherical symmetry the lensed sky seen by a static observer at radius r
+ * depends on one local angle psi to the radial axis: a fan of exact
+ * geodesics over psi in [0, pi] (SkyFanState) turns every display pixel
+ * into a table lookup plus a sky sample, and only the disc/gas is traced per
+ * pixel, into a premultiplied layer composited over the sky.
+ *
+ * An image is a (channels, w, h) float array in device memory,
+ * float[h][w][channels]; a layer has 4 channels (premultiplied RGB,
+ * transmittance to the sky). */
+typedef struct st_image st_image;
+int st_image_create(st_ctx* ctx, int channels, int w, int h, st_image** out);   /* zero-filled */
+int st_image_destroy(st_image* img);
+int st_image_download(st_ctx* ctx, const st_image* img, float* host, size_t nfloats);
+int st_image_upload(st_ctx* ctx, st_image* img, const float* host, size_t nfloats);
+/* SkyFanState (src/metal.jl:2957-2968): a coarse fan of n entries uniform in
+ * psi over [0, pi], a fine fan of n_fine entries across the escape/capture
+ * band, each entry (escaped, cos thf, sin thf, r_min), and the 12-float sky
+ * parameter block (0-3 radial tetrad axis e_r, 4 M, 5 r_escape, 6 gas gate
+ * radius, 8-9 the psi band the fine fan covers). */
+typedef struct st_skyfan st_skyfan;
+int st_skyfan_create(st_ctx* ctx, int n, int n_fine, st_skyfan** out);
+int st_skyfan_destroy(st_skyfan* fan);
+/* update_sky_fan! (src/metal.jl:2978-3028): the coarse fan
+ * (sky_fan_kernel!, src/metal.jl:2423-2519), the band (fan_band_kernel!,
+ * src/metal.jl:2888-2898, pad 3; should two pairs of escape flags differ,
+ * the larger psi) and the fine fan, all on the context's stream.  fan_cam[20] is the fan camera's block (position (r, 0, 0), 1,
+ * Ef, Er, Eu, u of the static tetrad with fwd x, right y); sky_params[12] as
+ * above with 8-11 zero (the band kernel fills 8-9). */
+int st_skyfan_update(st_ctx* ctx, st_skyfan* fan, const float fan_cam[20], const float sky_params[12], int nmax,
+                     float dt);
+/* 1 if the last st_skyfan_update integrated the fans, 0 if it reused them:
+ * the fans depend only on the fan camera block, M, r_escape, dt and nmax, so
+ * an update at an unchanged radius rewrites the sky block alone (identical
+ * results; the reference integrates every frame). */
+int st_skyfan_recomputed(const st_skyfan* fan);
+/* Speculative refinement fan (default on): the fine fan is integrated beside
+ * the coarse one over the band predicted from the critical angle and redone
+ * where the band the coarse fan yields differs, so the fans equal the
+ * sequential order's bit for bit; off runs coarse, band, fine in sequence.
+ * No counterpart in the reference. */
+int st_skyfan_set_speculate(st_skyfan* fan, int on);
+/* Copy the fans (float[n][4], float[n_fine][4]) and the sky block (12
+ * floats) to the host; any pointer may be null. */
+int st_skyfan_download(st_ctx* ctx, const st_skyfan* fan, float* coarse, float* fine, float* sky_params);
+/* The LAYER trace of render_layered_gpu! (src/metal.jl:3079-3113, the
+ * tracer's LAYER branches src/metal.jl:782-793, 1095-1135, 1202-1211,
+ * 1613-1627): premultiplied emission and transmittance written by
+ * assignment into `layer` (4 channels), rays the fan proves empty culled to
+ * transparency.  ring_rmin > 0 makes it the ring pass (only rays whose fan
+ * periapsis is below ring_rmin).  A sub-grid pass (substride > 1) traces
+ * every substride-th pixel from (subx, suby) on a sensor plane substride
+ * times the dispatch (cld(w, substride) x cld(h, substride)); args' tile
+ * (row0/rows, col0/cols) is in dispatch pixels and args->width/height must
+ * equal the layer's.  FP32 fixed-step (order 4, 2, 1).  Asynchronous. */
+int st_trace_layer(st_ctx* ctx, const st_skyfan* fan, st_image* layer, const float cam_params[49],
+                   const double* cam_params64, const float spacetime_params[8], const st_trace_args* args,
+                   float ring_rmin, int substride, int subx, int suby);
+/* sky_composite_kernel! (src/metal.jl:2707-2878): per display pixel of the
+ * context's (3, W, H) buffer, the exact fan-driven sky (fine fan inside the
+ * band, nearest entry where neighbours disagree, context sky texture and
+ * starfield along the asymptotic direction, black when captured) under the
+ * bilinear layer, the ring layer blended in on fan periapsis when ring is
+ * non-null and ring_rmin > 0.  accumulate != 0 adds, else assigns; rows < 0
+ * means to the bottom.  Asynchronous. */
+int st_sky_composite(st_ctx* ctx, const st_skyfan* fan, const st_image* layer, const st_image* ring,
+                     const float cam_params[49], const float spacetime_params[8], float ring_rmin, float ju,
+                     float jv, int accumulate, int row0, int rows);
 
 /* ---- live fluid-disc simulation (src/disc_sim.jl) ----
  * A 2D stable-fluids solver on the gas volume's (log s, phi) grid (Stam

@@ -81,6 +81,11 @@ _SIG = {
     "ora_warp_sample": (None, [C.POINTER(OraScene), _F, C.c_int, C.c_int, _F, C.c_int, C.c_int, _D, C.c_int,
                                C.c_int, _D]),
     "ora_tsit5_tableau": (None, [_D, _D, _D, _D]),
+    "ora_sky_fan": (None, [_D, _D, C.c_int, C.c_double, C.c_int, C.c_int, _D, _D]),
+    "ora_fan_band": (None, [_D, C.c_int, C.c_int, _D]),
+    "ora_trace_layer": (None, [C.POINTER(OraScene), C.c_int, C.c_int, _D, C.c_int, _D, C.c_double, _D]),
+    "ora_sky_composite": (None, [C.POINTER(OraScene), _D, C.c_int, _D, C.c_int, _D, _F, C.c_int, C.c_int, _F,
+                                 C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _D]),
 }
 
 _lib = None
@@ -295,6 +300,35 @@ class Scene:
                               out.ctypes.data_as(_D))
         return out
 
+    def trace_layer(self, i: int, j: int, fan: np.ndarray, skyp, ring_rmin: float = 0.0) -> np.ndarray:
+        """The LAYER output (premultiplied emission, transmittance) of pixel
+        (i, j) against a fan (n, 4) and sky block; this scene's width/height
+        are the sensor plane."""
+        f = np.ascontiguousarray(fan, dtype=np.float64)
+        sp = np.ascontiguousarray(skyp, dtype=np.float64)
+        out = np.zeros(4)
+        lib().ora_trace_layer(self.ptr, int(i), int(j), f.ctypes.data_as(_D), f.shape[0], sp.ctypes.data_as(_D),
+                              float(ring_rmin), out.ctypes.data_as(_D))
+        return out
+
+    def sky_composite(self, fan, fine, skyp, layer: np.ndarray, ring: np.ndarray | None = None,
+                      ring_rmin: float = 0.0, row0: int = 0, rows: int | None = None) -> np.ndarray:
+        """sky_composite_kernel! over this scene's frame (camera block, sky,
+        stars, ju/jv): (H, W, 3); layer/ring float32 (h, w, 4)."""
+        f = np.ascontiguousarray(fan, dtype=np.float64)
+        fi = np.ascontiguousarray(fine, dtype=np.float64)
+        sp = np.ascontiguousarray(skyp, dtype=np.float64)
+        la = np.ascontiguousarray(layer, dtype=np.float32)
+        ri = None if ring is None else np.ascontiguousarray(ring, dtype=np.float32)
+        rows = self.s.height - row0 if rows is None else rows
+        out = np.zeros((self.s.height, self.s.width, 3))
+        lib().ora_sky_composite(self.ptr, f.ctypes.data_as(_D), f.shape[0], fi.ctypes.data_as(_D), fi.shape[0],
+                                sp.ctypes.data_as(_D), la.ctypes.data_as(_F), la.shape[1], la.shape[0],
+                                None if ri is None else ri.ctypes.data_as(_F), 0 if ri is None else ri.shape[1],
+                                0 if ri is None else ri.shape[0], float(ring_rmin), int(row0), int(rows),
+                                out.ctypes.data_as(_D))
+        return out
+
     def render(self, method="rk4", tol=1e-6, row0=0, rows=None, nthreads=0, weight=1.0):
         rows = self.s.height - row0 if rows is None else rows
         out = np.zeros((self.s.height, self.s.width, 3), dtype=np.float32)
@@ -311,6 +345,25 @@ def warp_lookup(m: np.ndarray, d, cphi: float, sphi: float) -> np.ndarray:
     lib().ora_warp_lookup(m.ctypes.data_as(_F), m.shape[1], m.shape[0], dd.ctypes.data_as(_D), float(cphi),
                           float(sphi), out.ctypes.data_as(_D))
     return out
+
+
+def sky_fan(cam20, skyp, nmax: int, dt: float, n: int, band: bool = False):
+    """sky_fan_kernel! in FP64: (fan (n, 4), ends (n, 4) = (|x|, x, y, steps))."""
+    c = np.ascontiguousarray(cam20, dtype=np.float64)
+    sp = np.ascontiguousarray(skyp, dtype=np.float64)
+    fan = np.zeros((n, 4))
+    ends = np.zeros((n, 4))
+    lib().ora_sky_fan(c.ctypes.data_as(_D), sp.ctypes.data_as(_D), int(nmax), float(dt), int(band), int(n),
+                      fan.ctypes.data_as(_D), ends.ctypes.data_as(_D))
+    return fan, ends
+
+
+def fan_band(fan: np.ndarray, skyp, pad: int = 3) -> np.ndarray:
+    """fan_band_kernel!: a copy of skyp with the band in slots 8-9."""
+    f = np.ascontiguousarray(fan, dtype=np.float64)
+    sp = np.array(skyp, dtype=np.float64)
+    lib().ora_fan_band(f.ctypes.data_as(_D), f.shape[0], int(pad), sp.ctypes.data_as(_D))
+    return sp
 
 
 REC_FIELDS = ["fate", "steps", "dx", "dy", "dz", "r", "g", "b", "ham", "r_end", "rk_min", "ncross",

@@ -847,21 +847,59 @@ extern "C" {
  * rule), 1382-1401 (RK4; midpoint and Euler for order 2 and 1), 1278-1381
  * (adaptive Tsit5, order 45/46), 1414-1421 (NaN bail), 1440-1571 (gas, thin
  * disc), 1629-1704 (fate, sky). */
-static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets);
+/* The layered engine's LAYER pass (src/metal.jl:1095-1135, 1202-1211,
+ * 1613-1627): the fan (n entries of 4), the sky block and the ring pass's
+ * periapsis bound. */
+struct LayerIn {
+    const double* fan;
+    int n;
+    const double* skyp;
+    double ring_rmin;
+    double out[4];
+};
 
-int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) { return trace_rk4(sc, i, j, o, 0, nullptr); }
+static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets, LayerIn* lay);
+
+int ora_trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o) {
+    return trace_rk4(sc, i, j, o, 0, nullptr, nullptr);
+}
 
 int ora_trace_depth(const ora_scene* sc, int i, int j, int nb, double* buckets, ora_ray* o) {
     for (int k = 0; k < 3 * nb; ++k) buckets[k] = 0.0;
-    return trace_rk4(sc, i, j, o, nb, buckets);
+    return trace_rk4(sc, i, j, o, nb, buckets, nullptr);
 }
 
-static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets) {
+static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, double* buckets, LayerIn* lay) {
     RayStart rs;
     ray_start(sc, i, j, o, rs);
     const double M = sc->M;
     double s[6];
     std::memcpy(s, rs.s, sizeof(s));
+    /* layered mode: a ray whose fan periapsis (at its local angle psi to the
+     * radial axis e_r) proves it stays outside the gas gate is transparent
+     * without a step; the ring pass keeps only rays dipping below ring_rmin */
+    double gate = 0.0, r_layer_prev = -1.0;
+    if (lay) {
+        gate = lay->skyp[6];
+        const double rc = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        if (rc > gate || lay->ring_rmin > 0.0) {
+            const double cpsi = clampd(rs.p_t * lay->skyp[0] + s[3] * lay->skyp[1] + s[4] * lay->skyp[2] +
+                                       s[5] * lay->skyp[3], -1.0, 1.0);
+            const double tf = std::acos(cpsi) * (lay->n - 1.0) / M_PI;
+            const int k0 = std::min(std::max((int)tf, 0), lay->n - 2);
+            const double fr = tf - k0;
+            const double* f0 = lay->fan + 4 * k0;
+            const double esc = f0[0] + fr * (f0[4] - f0[0]);
+            const double rmin = f0[3] + fr * (f0[7] - f0[3]);
+            const bool cull = lay->ring_rmin > 0.0 ? rmin >= lay->ring_rmin : (esc > 0.999 && rmin > gate);
+            if (cull) {
+                lay->out[0] = lay->out[1] = lay->out[2] = 0.0;
+                lay->out[3] = 1.0;
+                o->fate = 0;
+                return 0;
+            }
+        }
+    }
     Shade sh;
     sh.nb = nb;
     sh.buckets = buckets;
@@ -889,6 +927,11 @@ static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, doub
         const double r = std::sqrt(r2);
         const double rk2 = ks_r2(sc, s);
         o->rk_min = std::min(o->rk_min, std::sqrt(std::max(rk2, 0.0)));
+        if (lay) {
+            /* outbound beyond the gate: at most one radial turning point */
+            if (r > gate && r_layer_prev > 0.0 && r > r_layer_prev) break;
+            r_layer_prev = r;
+        }
         if (sc->capture_mode == 1) {
             if ((rk2 < rs.rcap * rs.rcap && rk2 < rk2_prev) || rk2 < 0.09 * M * M) { hit = true; break; }
             rk2_prev = rk2;
@@ -1028,6 +1071,12 @@ static int trace_rk4(const ora_scene* sc, int i, int j, ora_ray* o, int nb, doub
         }
     }
     o->steps = steps;
+    if (lay) {
+        for (int c = 0; c < 3; ++c) lay->out[c] = sh.acc[c];
+        lay->out[3] = sh.alpha;
+        o->fate = 0;
+        return 0;
+    }
     const bool interp = sc->esc_interp && escaped_loop && have_prev;
     finish(sc, s, rs.p_t, rs.scam, hit, nonfinite, rs.captured0, interp ? vstart : nullptr,
            interp ? xstart : nullptr, sh, o);
@@ -1368,6 +1417,233 @@ void ora_warp_sample(const ora_scene* sc, const float* m0, int w0, int h0, const
             }
             double* o = out + ((size_t)j * width + i) * 3;
             for (int c = 0; c < 3; ++c) o[c] = (v[3 + c] + v[6] * sky[c]) * p[20];
+        }
+}
+
+/* ---------------------------------------------------------------------------
+ * Layered engine (src/metal.jl:2393-3142)
+ * ------------------------------------------------------------------------- */
+
+/* sky_fan_kernel! (src/metal.jl:2423-2519) in FP64: entry k integrates the
+ * Schwarzschild geodesic from the fan camera (r, 0, 0) at local angle psi_k
+ * (pi k/(n-1), or across skyp[8..9] when band) with the kernel's RK4, step
+ * rule and kill tests; fan[k] = (escaped, cos thf, sin thf, r_min).  ends,
+ * when non-null, receives (|x|, x, y, steps) of each final point. */
+void ora_sky_fan(const double cam[20], const double skyp[12], int nmax, double dt, int band, int n, double* fan,
+                 double* ends) {
+    const double M = skyp[4], r_escape = skyp[5];
+    for (int k = 0; k < n; ++k) {
+        const double psi = band > 0 ? skyp[8] + (skyp[9] - skyp[8]) * k / (n - 1.0) : M_PI * k / (n - 1.0);
+        const double cf = std::cos(psi), cr = std::sin(psi);
+        double s[6] = {cam[0], cam[1], cam[2], 0, 0, 0};
+        double r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        const double f = 2.0 * M / r;
+        double q[4];
+        for (int c = 0; c < 4; ++c) q[c] = cf * cam[4 + c] + cr * cam[8 + c] - cam[16 + c];
+        const double lq = q[0] + (s[0] * q[1] + s[1] * q[2] + s[2] * q[3]) / r;
+        const double p_t = -q[0] + f * lq;
+        for (int c = 0; c < 3; ++c) s[3 + c] = q[1 + c] + f * lq / r * s[c];
+        double r_min = r, r_prev = -1.0;
+        bool hit = false;
+        const double r_floor = r > 2.05 * M ? 2.0 * M : 0.3 * M;
+        int steps = 0;
+        for (int it = 0; it < nmax; ++it) {
+            r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            r_min = std::min(r_min, r);
+            if (r < 3.2 * M) {
+                if (r < r_floor || (r < 2.95 * M && r_prev > 0.0 && r < r_prev - 1e-4 * M)) { hit = true; break; }
+                r_prev = r;
+            }
+            if (r > r_escape) break;
+            const double h = dt * std::min(std::max(0.16 * r / M, 1.0), 8.0);
+            double k1[6], k2[6], k3[6], k4[6], t[6];
+            ora_rhs(0, s, p_t, M, 0.0, k1);
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k1[c];
+            ora_rhs(0, t, p_t, M, 0.0, k2);
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + 0.5 * h * k2[c];
+            ora_rhs(0, t, p_t, M, 0.0, k3);
+            for (int c = 0; c < 6; ++c) t[c] = s[c] + h * k3[c];
+            ora_rhs(0, t, p_t, M, 0.0, k4);
+            for (int c = 0; c < 6; ++c) s[c] += (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
+            ++steps;
+            if (!(s[0] == s[0]) || !(s[3] == s[3])) { hit = true; break; }
+        }
+        const double rf = std::max(std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]), 1e-6);
+        double* o = fan + 4 * (size_t)k;
+        if (hit || rf < 4.0 * M) {
+            o[0] = 0.0; o[1] = 1.0; o[2] = 0.0;
+        } else {
+            /* the coordinate velocity's in-plane direction at the end */
+            const double ff = 2.0 * M / rf;
+            const double kf = (s[0] * s[3] + s[1] * s[4] + s[2] * s[5]) / rf;
+            const double c1 = ff * (-p_t + kf) / rf;
+            const double vx = s[3] - c1 * s[0], vy = s[4] - c1 * s[1];
+            const double vl = std::max(std::sqrt(vx * vx + vy * vy), 1e-20);
+            o[0] = 1.0; o[1] = vx / vl; o[2] = vy / vl;
+        }
+        o[3] = r_min;
+        if (ends) {
+            double* e = ends + 4 * (size_t)k;
+            e[0] = rf; e[1] = s[0]; e[2] = s[1]; e[3] = steps;
+        }
+    }
+}
+
+/* fan_band_kernel! (src/metal.jl:2888-2898): the psi band [transition - pad,
+ * transition + pad] (coarse spacings) into skyp[8..9]; the last differing
+ * pair wins, as the serial order of a race the kernel never sees (the escape
+ * flag is monotone in psi). */
+void ora_fan_band(const double* fan, int n, int pad, double skyp[12]) {
+    const double dpsi = M_PI / (n - 1.0);
+    for (int k = 0; k < n - 1; ++k)
+        if (fan[4 * k] != fan[4 * (k + 1)]) {
+            skyp[8] = std::max((double)(k - pad), 0.0) * dpsi;
+            skyp[9] = std::min((double)(k + 1 + pad), n - 1.0) * dpsi;
+        }
+}
+
+/* The LAYER trace of one pixel (src/metal.jl:1095-1135, 1202-1211,
+ * 1613-1627): out[4] = premultiplied emission, transmittance.  The scene's
+ * width/height are the sensor plane. */
+void ora_trace_layer(const ora_scene* sc, int i, int j, const double* fan, int n, const double skyp[12],
+                     double ring_rmin, double out[4]) {
+    LayerIn lay{fan, n, skyp, ring_rmin, {0, 0, 0, 1}};
+    ora_ray o;
+    trace_rk4(sc, i, j, &o, 0, nullptr, &lay);
+    for (int c = 0; c < 4; ++c) out[c] = lay.out[c];
+}
+
+}  // extern "C"
+
+namespace {
+
+/* _fan_dir (src/metal.jl:2905-2935) */
+void fan_dir(const double* fan, int n, const double* fine, int nf, double lo, double hi, double psi, double out[3]) {
+    const bool use_fine = hi > lo && lo <= psi && psi <= hi;
+    const double* t = use_fine ? fine : fan;
+    const int m = use_fine ? nf : n;
+    const double tf = use_fine ? (psi - lo) / (hi - lo) * (nf - 1.0) : psi * (n - 1.0) / M_PI;
+    const int k0 = std::min(std::max((int)tf, 0), m - 2);
+    const double fr = tf - k0;
+    const double* a = t + 4 * k0;
+    const double* b = a + 4;
+    if (a[0] == b[0] && a[1] * b[1] + a[2] * b[2] > 0.9987) {
+        const double c = a[1] + fr * (b[1] - a[1]), s = a[2] + fr * (b[2] - a[2]);
+        const double nl = std::max(std::sqrt(c * c + s * s), 1e-6);
+        out[0] = a[0]; out[1] = c / nl; out[2] = s / nl;
+        return;
+    }
+    const double* q = fr < 0.5 ? a : b;
+    out[0] = q[0]; out[1] = q[1]; out[2] = q[2];
+}
+
+/* _layer_bilinear (src/metal.jl:2670-2691), 0-based pixel (i, j) */
+void layer_tap(const float* L, int i, int j, int W, int H, int lw, int lh, double out[4]) {
+    const double fx = (i + 0.5) * lw / W + 0.5, fy = (j + 0.5) * lh / H + 0.5;
+    const int x0 = std::min(std::max((int)std::floor(fx), 1), lw - 1);
+    const int y0 = std::min(std::max((int)std::floor(fy), 1), lh - 1);
+    const double tx = clampd(fx - x0, 0.0, 1.0), ty = clampd(fy - y0, 0.0, 1.0);
+    const double w[4] = {(1 - tx) * (1 - ty), tx * (1 - ty), (1 - tx) * ty, tx * ty};
+    const float* p[4] = {L + 4 * ((size_t)(y0 - 1) * lw + (x0 - 1)), L + 4 * ((size_t)(y0 - 1) * lw + x0),
+                         L + 4 * ((size_t)y0 * lw + (x0 - 1)), L + 4 * ((size_t)y0 * lw + x0)};
+    for (int c = 0; c < 4; ++c) out[c] = p[0][c] * w[0] + p[1][c] * w[1] + p[2][c] * w[2] + p[3][c] * w[3];
+}
+
+}  // namespace
+
+extern "C" {
+
+/* sky_composite_kernel! (src/metal.jl:2707-2878) over rows [row0, row0+rows)
+ * of the scene's width x height frame: out[H][W][3] assigned.  The scene
+ * supplies the camera block, M, the relativistic flag, sky and starfield;
+ * layer/ring are float[h][w][4]; ring null or ring_rmin <= 0: no ring. */
+void ora_sky_composite(const ora_scene* sc, const double* fan, int n, const double* fine, int nf,
+                       const double skyp[12], const float* layer, int lw, int lh, const float* ring, int rw, int rh,
+                       double ring_rmin, int row0, int rows, double* out) {
+    const double* c = sc->cam;
+    const double M = sc->M;
+    const int W = sc->width, H = sc->height;
+    const bool stars = sc->star_params[0] > 0.0 && sc->star_lut;
+    const bool skytex = !(sc->star_params[0] > 0.0) || sc->star_params[15] > 0.0;
+    for (int j = row0; j < row0 + rows; ++j)
+        for (int i = 0; i < W; ++i) {
+            const double hh = H / 2.0;
+            const double u = (i + sc->ju - W / 2.0) / hh, v = (j + sc->jv - H / 2.0) / hh;
+            double cr, cu, cf;
+            if (c[20] > 0.5) {
+                const double rho = std::sqrt(u * u + v * v), tp = rho * c[21], st = std::sin(tp);
+                const double ir = rho > 1e-8 ? 1.0 / rho : 0.0;
+                cr = st * u * ir; cu = st * v * ir; cf = std::cos(tp);
+            } else {
+                const double dx = u * c[3], dy = v * c[3], nu = std::sqrt(dx * dx + dy * dy + 1.0);
+                cr = dx / nu; cu = dy / nu; cf = 1.0 / nu;
+            }
+            const double x = c[0], y = c[1], z = c[2];
+            const double r = std::sqrt(x * x + y * y + z * z), f = 2.0 * M / r;
+            double q[4];
+            for (int m = 0; m < 4; ++m) q[m] = cf * c[4 + m] + cr * c[8 + m] + cu * c[12 + m] - c[16 + m];
+            const double lq = q[0] + (x * q[1] + y * q[2] + z * q[3]) / r;
+            const double p_t = -q[0] + f * lq;
+            const double px = q[1] + f * lq / r * x, py = q[2] + f * lq / r * y, pz = q[3] + f * lq / r * z;
+            const double scam = sc->relativistic ? 1.0 / clampd(std::fabs(p_t), 0.05, 20.0) : 1.0;
+            const double cpsi = clampd(p_t * skyp[0] + px * skyp[1] + py * skyp[2] + pz * skyp[3], -1.0, 1.0);
+            const double psi = std::acos(cpsi);
+            const double tpsi = psi * (n - 1.0) / M_PI;
+            double fd[3];
+            fan_dir(fan, n, fine, nf, skyp[8], skyp[9], psi, fd);
+            double sky[3] = {0, 0, 0};
+            if (fd[0] > 0.5) {
+                /* asymptotic direction in the ray's own plane */
+                const double k0 = (x * px + y * py + z * pz) / r;
+                const double c10 = f * (-p_t + k0) / r;
+                const double vx = px - c10 * x, vy = py - c10 * y, vz = pz - c10 * z;
+                const double e1[3] = {x / r, y / r, z / r};
+                const double vr = vx * e1[0] + vy * e1[1] + vz * e1[2];
+                double l[3] = {vx - vr * e1[0], vy - vr * e1[1], vz - vr * e1[2]};
+                const double ll = std::sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+                if (ll < 1e-7) {
+                    l[0] = -e1[1]; l[1] = e1[0]; l[2] = 0.0;
+                    const double lm = std::max(std::sqrt(l[0] * l[0] + l[1] * l[1]), 1e-6);
+                    l[0] /= lm; l[1] /= lm;
+                } else {
+                    for (int m = 0; m < 3; ++m) l[m] /= ll;
+                }
+                double d[3];
+                for (int m = 0; m < 3; ++m) d[m] = fd[1] * e1[m] + fd[2] * l[m];
+                const double dl = std::max(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), 1e-20);
+                if (skytex && sc->sky)
+                    ora_sample_background(sc->sky, sc->sky_w, sc->sky_h,
+                                          std::acos(clampd(d[2] / std::max(dl, 1e-9), -1.0, 1.0)),
+                                          std::atan2(d[1], d[0]), sky);
+                if (stars) {
+                    if (skytex)
+                        for (int m = 0; m < 3; ++m) sky[m] *= sc->star_params[15];
+                    double st[3];
+                    ora_starfield(sc, d[0] / dl, d[1] / dl, d[2] / dl, st);
+                    for (int m = 0; m < 3; ++m) sky[m] += sc->star_params[0] * st[m];
+                }
+                if (scam != 1.0) {
+                    sky[0] *= 57.4 / (std::exp(4.067 / scam) - 1.0);
+                    sky[1] *= 90.2 / (std::exp(4.513 / scam) - 1.0);
+                    sky[2] *= 206.5 / (std::exp(5.335 / scam) - 1.0);
+                }
+            }
+            double Lc[4];
+            layer_tap(layer, i, j, W, H, lw, lh, Lc);
+            if (ring && ring_rmin > 0.0) {
+                const int rk0 = std::min(std::max((int)tpsi, 0), n - 2);
+                const double rfr = tpsi - rk0;
+                const double rmin = fan[4 * rk0 + 3] + rfr * (fan[4 * (rk0 + 1) + 3] - fan[4 * rk0 + 3]);
+                const double t = clampd((rmin - 0.8 * ring_rmin) / std::max(0.2 * ring_rmin, 1e-6), 0.0, 1.0);
+                const double wr = 1.0 - t * t * (3.0 - 2.0 * t);
+                if (wr > 0.0) {
+                    double R[4];
+                    layer_tap(ring, i, j, W, H, rw, rh, R);
+                    for (int m = 0; m < 4; ++m) Lc[m] += wr * (R[m] - Lc[m]);
+                }
+            }
+            double* o = out + ((size_t)j * W + i) * 3;
+            for (int m = 0; m < 3; ++m) o[m] = Lc[m] + Lc[3] * sky[m];
         }
 }
 

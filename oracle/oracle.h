@@ -21,6 +21,8 @@
  *   - procedural starfield, _sim_hash      src/metal.jl:652-724, src/disc_sim.jl:83-91
  *   - blackbody_rgb + LUT                  src/blackbody.jl:15-80
  *   - hamiltonian / metric_inverse         src/gr.jl:119-203
+ *   - layered engine: sky fan, band, LAYER trace, composite
+ *                                          src/metal.jl:2423-2935, 1095-1135, 1613-1627
  * and adds an adaptive Tsitouras 5(4) integrator (coefficients from
  * Tsitouras 2011, the tableau the kernel's ORD 45 path carries at
  * src/metal.jl:1278-1381) with exact event location, used as the
@@ -158,6 +160,25 @@ void ora_warp_lookup(const float* map, int mapw, int maph, const double d[3], do
  * starfield; p is the 24-float parameter block (0-based, as st_warp_sample). */
 void ora_warp_sample(const ora_scene* sc, const float* m0, int w0, int h0, const float* m1, int w1, int h1,
                      const double p[24], int width, int height, double* out);
+
+/* Layered engine (src/metal.jl:2393-3142).
+ * sky_fan_kernel! (src/metal.jl:2423-2519): fan[n][4] = (escaped, cos thf,
+ * sin thf, r_min) from the fan camera block cam[20]; band != 0 spans
+ * skyp[8..9], else [0, pi]; ends[n][4] (may be null) = (|x|, x, y, steps)
+ * of each final point. */
+void ora_sky_fan(const double cam[20], const double skyp[12], int nmax, double dt, int band, int n, double* fan,
+                 double* ends);
+/* fan_band_kernel! (src/metal.jl:2888-2898): the band into skyp[8..9]. */
+void ora_fan_band(const double* fan, int n, int pad, double skyp[12]);
+/* The tracer's LAYER output for pixel (i, j): out[4] = premultiplied emission,
+ * transmittance (src/metal.jl:1095-1135, 1202-1211, 1613-1627). */
+void ora_trace_layer(const ora_scene* sc, int i, int j, const double* fan, int n, const double skyp[12],
+                     double ring_rmin, double out[4]);
+/* sky_composite_kernel! (src/metal.jl:2707-2878) over rows [row0, row0+rows)
+ * of the scene's frame: out[H][W][3] assigned.  layer/ring float[h][w][4]. */
+void ora_sky_composite(const ora_scene* sc, const double* fan, int n, const double* fine, int nf,
+                       const double skyp[12], const float* layer, int lw, int lh, const float* ring, int rw, int rh,
+                       double ring_rmin, int row0, int rows, double* out);
 
 /* Tsitouras 5(4) tableau as used by the oracle: c[7], a[7][7], b[7], btilde[7]. */
 void ora_tsit5_tableau(double c[7], double a[49], double b[7], double btilde[7]);

@@ -115,6 +115,20 @@ SIGNATURES = {
     "st_map_download": (C.c_int, [C.c_void_p, C.c_void_p, _F, C.c_size_t]),
     "st_map_upload": (C.c_int, [C.c_void_p, C.c_void_p, _F, C.c_size_t]),
     "st_warp_sample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _F]),
+    "st_image_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "st_image_destroy": (C.c_int, [C.c_void_p]),
+    "st_image_download": (C.c_int, [C.c_void_p, C.c_void_p, _F, C.c_size_t]),
+    "st_image_upload": (C.c_int, [C.c_void_p, C.c_void_p, _F, C.c_size_t]),
+    "st_skyfan_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "st_skyfan_destroy": (C.c_int, [C.c_void_p]),
+    "st_skyfan_update": (C.c_int, [C.c_void_p, C.c_void_p, _F, _F, C.c_int, C.c_float]),
+    "st_skyfan_download": (C.c_int, [C.c_void_p, C.c_void_p, _F, _F, _F]),
+    "st_skyfan_recomputed": (C.c_int, [C.c_void_p]),
+    "st_skyfan_set_speculate": (C.c_int, [C.c_void_p, C.c_int]),
+    "st_trace_layer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _F, _D, _F, C.POINTER(TraceArgs), C.c_float,
+                                 C.c_int, C.c_int, C.c_int]),
+    "st_sky_composite": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _F, _F, C.c_float, C.c_float,
+                                   C.c_float, C.c_int, C.c_int, C.c_int]),
     "st_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_char_p, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "st_mlp_param_count": (C.c_size_t, []),

@@ -17,7 +17,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 LIB = PKG / "libspacetime_hip.so"
 
-SOURCES = ["st_kernels.hip", "st_kernels_adapt.hip", "st_kernels_depth.hip", "st_api.hip", "st_warp.hip",
+SOURCES = ["st_kernels.hip", "st_kernels_adapt.hip", "st_kernels_depth.hip", "st_api.hip", "st_warp.hip", "st_layer.hip",
            "st_fluid.hip", "st_mlp_kernels.hip", "st_mlp_api.hip", "st_host.cpp"]
 ARCH = os.environ.get("SPACETIME_ARCH", "gfx950")
 

@@ -1,0 +1,671 @@
+// Layered engine (src/metal.jl:2393-3142): by spherical symmetry the whole
+// lensed sky seen by a static observer at radius r is a function of one local
+// angle psi to the outward radial axis, so each frame integrates an exact fan
+// of a few thousand geodesics over psi in [0, pi] (plus a refinement fan
+// across the escape/capture transition) and every display pixel becomes a
+// table lookup plus a sky sample; only the disc/gas, which breaks the
+// symmetry, is traced per pixel, as a premultiplied layer (its own
+// resolution, rays the fan proves empty culled) composited over the sky.
+//
+//  k_sky_fans       one thread per fan entry, coarse and (speculative) fine
+//                   fans in one grid: Schwarzschild RK4 from (r, 0, 0)
+//  k_fan_band_fix   the transition's psi band, the fine fan redone if the
+//                   speculation missed
+//  k_trace_layer    the tracer's LAYER variant into a 4-channel layer
+//  k_sky_composite  per display pixel: ray, psi, fan lookup, sky, composite
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "st_kernels.h"
+
+namespace st {
+
+namespace {
+
+constexpr float kRingFeather = 0.8f;   // RING_FEATHER, src/metal.jl:2532
+
+// Arguments of the two fan launches.  The fan camera block (position
+// (r, 0, 0), 1, Ef, Er, Eu, u), M and r_escape travel by value.
+struct FanArgs {
+    float cam[20];
+    float M, r_escape, dt;
+    int nmax;
+    float* fan;        // float4 per coarse entry
+    float* fine;       // float4 per fine entry
+    int N, Nf;
+    int nbc;           // blocks of the coarse fan in the first launch
+    float pred_lo, pred_hi;   // speculative band of the fine fan
+    int pad;
+    float* skyp;
+    float sky[12];     // the sky block, band slots zero
+    int write_sky;     // 1: the first launch writes the sky block
+};
+
+// One fan entry: sky_fan_kernel!'s body (src/metal.jl:2423-2519) at local
+// angle psi — Schwarzschild RK4 from the fan camera under the step rule
+// h = dt clamp(0.16 r/M, 1, 8), the photon-sphere/horizon kill tests, then
+// (escaped, cos thf, sin thf, r_min).
+__device__ float4 fan_entry(const FanArgs& A, float psi) {
+    const float M = A.M;
+    const float cf = cosf(psi), cr = sinf(psi);
+    const float* c = A.cam;
+    S6<float> s;
+    s.x = c[0]; s.y = c[1]; s.z = c[2];
+    float r = sqrtf(s.x * s.x + s.y * s.y + s.z * s.z);
+    const float f = 2.0f * M / r;
+    const float qt = cf * c[4] + cr * c[8] - c[16];
+    const float qx = cf * c[5] + cr * c[9] - c[17];
+    const float qy = cf * c[6] + cr * c[10] - c[18];
+    const float qz = cf * c[7] + cr * c[11] - c[19];
+    const float lq = qt + (s.x * qx + s.y * qy + s.z * qz) / r;
+    const float pt = -qt + f * lq;
+    const float flr = f * lq / r;
+    s.px = qx + flr * s.x; s.py = qy + flr * s.y; s.pz = qz + flr * s.z;
+    float r_min = r;
+    bool hit = false;
+    const float r_floor = r > 2.05f * M ? 2.0f * M : 0.3f * M;
+    float r_prev = -1.0f;
+    for (int it = 0; it < A.nmax; ++it) {
+        r = sqrtf(s.x * s.x + s.y * s.y + s.z * s.z);
+        r_min = fminf(r_min, r);
+        if (r < 3.2f * M) {
+            if (r < r_floor || (r < 2.95f * M && r_prev > 0.0f && r < r_prev - 1e-4f * M)) {
+                hit = true;
+                break;
+            }
+            r_prev = r;
+        }
+        if (r > A.r_escape) break;
+        const float h = A.dt * fminf(fmaxf(0.16f * r / M, 1.0f), 8.0f);
+        float rk2;
+        const S6<float> k1 = rhs<float, false>(s, pt, M, 0.0f, rk2);
+        const S6<float> k2 = rhs<float, false>(axpy(s, 0.5f * h, k1), pt, M, 0.0f, rk2);
+        const S6<float> k3 = rhs<float, false>(axpy(s, 0.5f * h, k2), pt, M, 0.0f, rk2);
+        const S6<float> k4 = rhs<float, false>(axpy(s, h, k3), pt, M, 0.0f, rk2);
+        S6<float> acc = k1;
+        acc = axpy(acc, 2.0f, k2);
+        acc = axpy(acc, 2.0f, k3);
+        acc = axpy(acc, 1.0f, k4);
+        s = axpy(s, h / 6.0f, acc);
+        if (!(s.x == s.x) || !(s.px == s.px)) {
+            hit = true;
+            break;
+        }
+    }
+    const float rf = fmaxf(sqrtf(s.x * s.x + s.y * s.y + s.z * s.z), 1e-6f);
+    if (hit || rf < 4.0f * M) return make_float4(0.0f, 1.0f, 0.0f, r_min);
+    const float inv_rf = 1.0f / rf;
+    const float ff = 2.0f * M * inv_rf;
+    const float kf = (s.x * s.px + s.y * s.py + s.z * s.pz) * inv_rf;
+    const float c1f = ff * (-pt + kf) * inv_rf;
+    const float vx = s.px - c1f * s.x, vy = s.py - c1f * s.y;
+    const float vl = fmaxf(sqrtf(vx * vx + vy * vy), 1e-20f);
+    return make_float4(1.0f, vx / vl, vy / vl, r_min);
+}
+
+// The sky block's first 8 floats alone (radial axis, M, r_escape, gate).
+__global__ void k_sky_head(FanArgs A) {
+    if (threadIdx.x < 8) A.skyp[threadIdx.x] = A.sky[threadIdx.x];
+}
+
+__device__ __forceinline__ float fine_psi(float lo, float hi, int k, int Nf) {
+    return lo + (hi - lo) * (float)k / ((float)Nf - 1.0f);
+}
+
+// First launch: the coarse fan (psi = pi k/(N-1)) and, in the same grid, the
+// refinement fan over a band predicted from the critical angle.  The fan is
+// a few thousand serial integrations, latency-bound on a chip of 1024 SIMDs,
+// so running the two fans side by side instead of one after the other halves
+// the update; the second launch checks the prediction.
+__global__ __launch_bounds__(256) void k_sky_fans(FanArgs A) {
+    if (A.write_sky && blockIdx.x == 0 && threadIdx.x < 12) A.skyp[threadIdx.x] = A.sky[threadIdx.x];
+    if ((int)blockIdx.x < A.nbc) {
+        const int k = blockIdx.x * 256 + threadIdx.x;
+        if (k >= A.N) return;
+        reinterpret_cast<float4*>(A.fan)[k] = fan_entry(A, 3.14159265f * (float)k / ((float)A.N - 1.0f));
+    } else {
+        const int k = (blockIdx.x - A.nbc) * 256 + threadIdx.x;
+        if (k >= A.Nf) return;
+        reinterpret_cast<float4*>(A.fine)[k] = fan_entry(A, fine_psi(A.pred_lo, A.pred_hi, k, A.Nf));
+    }
+}
+
+// Second launch: fan_band_kernel! (src/metal.jl:2888-2898) — the band
+// [k - pad, k + 1 + pad] spacings around the pair k whose escape flags
+// differ (the flag is monotone in psi; should rounding break that, the
+// largest such k) — computed redundantly by every block; block 0 stores it.
+// Where it differs from the predicted band, each block re-integrates its
+// fine entries over the true band, so the result is the sequential one.
+__global__ __launch_bounds__(256) void k_fan_band_fix(FanArgs A) {
+    __shared__ int kmax_s[4];
+    int kmax = -1;
+    for (int k = threadIdx.x; k < A.N - 1; k += 256)
+        if (A.fan[4 * k] != A.fan[4 * (k + 1)]) kmax = k;   // k ascends per thread
+    for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off, 64));
+    if ((threadIdx.x & 63) == 0) kmax_s[threadIdx.x >> 6] = kmax;
+    __syncthreads();
+    kmax = max(max(kmax_s[0], kmax_s[1]), max(kmax_s[2], kmax_s[3]));
+    float lo = 0.0f, hi = 0.0f;    // no transition: the band slots keep their zeros
+    if (kmax >= 0) {
+        const float dpsi = 3.14159265f / ((float)A.N - 1.0f);
+        lo = fmaxf((float)kmax - (float)A.pad, 0.0f) * dpsi;
+        hi = fminf((float)(kmax + 1) + (float)A.pad, (float)A.N - 1.0f) * dpsi;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            A.skyp[8] = lo;
+            A.skyp[9] = hi;
+        }
+    }
+    if (lo == A.pred_lo && hi == A.pred_hi) return;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= A.Nf) return;
+    reinterpret_cast<float4*>(A.fine)[k] = fan_entry(A, fine_psi(lo, hi, k, A.Nf));
+}
+
+// Tracer LAYER variant into a (4, lw, lh) layer by assignment; a sub-grid
+// pass (substride > 1, src/metal.jl:782-793) traces every substride-th pixel
+// of the layer from offset (subx, suby), leaving the rest untouched.  The
+// sensor plane (F.width x F.height) is the dispatch times substride, as in the
+// reference; pixels of it beyond the layer are not written.
+template <bool KERR, bool VOL>
+__global__ __launch_bounds__(256) void k_trace_layer(const DeviceScene* __restrict__ Dp, TraceParams P, FrameRef F,
+                                                     TileGeom G, int lw, int lh, int substride, int subx, int suby) {
+    const DeviceScene& D = *Dp;
+    int di, dj;
+    bool valid;
+    tile_pixel(G, blockIdx.x, threadIdx.x, di, dj, valid);
+    const int i = di * substride + subx, j = dj * substride + suby;
+    valid = valid && i < lw && j < lh;
+    unsigned long long steps = 0;
+    if (valid) {
+        RayOut<float> o;
+        trace_ray<float, KERR, VOL, false, false, false, true>(D, P, F, i, j, o);
+        reinterpret_cast<float4*>(F.out)[(size_t)j * lw + i] = make_float4(o.rgb[0], o.rgb[1], o.rgb[2], o.alpha);
+        steps = (unsigned long long)o.steps;
+    }
+    const unsigned long long tot = wave_sum(steps);
+    const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0) {
+        if (tot) atomicAdd(&P.counters[0], tot);
+        if (nv) atomicAdd(&P.counters[3], nv);
+    }
+}
+
+// _fan_dir (src/metal.jl:2905-2935): (escaped, cos, sin) at psi from the fine
+// fan inside the critical band, else the coarse one; where neighbouring
+// entries wind or disagree, the nearest entry.
+__device__ __forceinline__ void fan_dir(const float4* __restrict__ fan, int N, const float4* __restrict__ fine, int Nf,
+                                        float lo, float hi, float psi, float& esc, float& ct, float& st) {
+    const bool use_fine = hi > lo && lo <= psi && psi <= hi;
+    const float4* t = use_fine ? fine : fan;
+    const int n = use_fine ? Nf : N;
+    const float tf = use_fine ? (psi - lo) / (hi - lo) * ((float)Nf - 1.0f) : psi * ((float)N - 1.0f) / 3.14159265f;
+    int k0 = (int)tf;
+    k0 = k0 < 0 ? 0 : (k0 > n - 2 ? n - 2 : k0);
+    const float fr = tf - (float)k0;
+    const float4 a = t[k0], b = t[k0 + 1];
+    if (a.x == b.x && a.y * b.y + a.z * b.z > 0.9987f) {
+        const float c = a.y + fr * (b.y - a.y), s = a.z + fr * (b.z - a.z);
+        const float nl = fmaxf(sqrtf(c * c + s * s), 1e-6f);
+        esc = a.x; ct = c / nl; st = s / nl;
+        return;
+    }
+    const float4 q = fr < 0.5f ? a : b;
+    esc = q.x; ct = q.y; st = q.z;
+}
+
+// _layer_bilinear (src/metal.jl:2670-2691), pixel (i, j) 0-based
+__device__ __forceinline__ float4 layer_tap(const float4* __restrict__ L, int i, int j, int W, int H, int lw, int lh) {
+    const float fx = ((float)i + 0.5f) * (float)lw / (float)W + 0.5f;
+    const float fy = ((float)j + 0.5f) * (float)lh / (float)H + 0.5f;
+    int x0 = (int)floorf(fx), y0 = (int)floorf(fy);
+    x0 = x0 < 1 ? 1 : (x0 > lw - 1 ? lw - 1 : x0);
+    y0 = y0 < 1 ? 1 : (y0 > lh - 1 ? lh - 1 : y0);
+    const float tx = fminf(fmaxf(fx - (float)x0, 0.0f), 1.0f);
+    const float ty = fminf(fmaxf(fy - (float)y0, 0.0f), 1.0f);
+    const float w00 = (1.0f - tx) * (1.0f - ty), w10 = tx * (1.0f - ty), w01 = (1.0f - tx) * ty, w11 = tx * ty;
+    // 1-based (x0, y0) -> 0-based memory
+    const float4 a = L[(size_t)(y0 - 1) * lw + (x0 - 1)], b = L[(size_t)(y0 - 1) * lw + x0];
+    const float4 c = L[(size_t)y0 * lw + (x0 - 1)], d = L[(size_t)y0 * lw + x0];
+    return make_float4(a.x * w00 + b.x * w10 + c.x * w01 + d.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + d.y * w11,
+                       a.z * w00 + b.z * w10 + c.z * w01 + d.z * w11, a.w * w00 + b.w * w10 + c.w * w01 + d.w * w11);
+}
+
+struct CompositeArgs {
+    float cam[22];
+    float M;
+    int relativistic;
+    const float4* fan;
+    const float4* fine;
+    int N, Nf;
+    const float* skyp;
+    const float4* layer;
+    const float4* ring;
+    int lw, lh, rw, rh;
+    float ring_rmin;
+    float ju, jv;
+    int accumulate;
+    int width, height, row0, rows;
+    float* out;
+};
+
+// sky_composite_kernel! (src/metal.jl:2707-2878)
+__global__ __launch_bounds__(256) void k_sky_composite(const DeviceScene* __restrict__ Dp, CompositeArgs A) {
+    const DeviceScene& D = *Dp;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= A.width * A.rows) return;
+    const int j = idx / A.width + A.row0, i = idx % A.width;
+    if (j >= A.height) return;
+    const float* c = A.cam;
+    const float M = A.M;
+    const float half_h = (float)A.height / 2.0f;
+    const float u = ((float)i + A.ju - (float)A.width / 2.0f) / half_h;
+    const float v = ((float)j + A.jv - (float)A.height / 2.0f) / half_h;
+    float cr, cu, cf;
+    if (c[20] > 0.5f) {
+        const float rho = sqrtf(u * u + v * v);
+        const float tp = rho * c[21];
+        const float sth = sinf(tp);
+        const float inv_rho = rho > 1e-8f ? 1.0f / rho : 0.0f;
+        cr = sth * u * inv_rho; cu = sth * v * inv_rho; cf = cosf(tp);
+    } else {
+        const float dxl = u * c[3], dyl = v * c[3];
+        const float nu = sqrtf(dxl * dxl + dyl * dyl + 1.0f);
+        cr = dxl / nu; cu = dyl / nu; cf = 1.0f / nu;
+    }
+    const float x = c[0], y = c[1], z = c[2];
+    const float r = sqrtf(x * x + y * y + z * z);
+    const float f = 2.0f * M / r;
+    const float qt = cf * c[4] + cr * c[8] + cu * c[12] - c[16];
+    const float qx = cf * c[5] + cr * c[9] + cu * c[13] - c[17];
+    const float qy = cf * c[6] + cr * c[10] + cu * c[14] - c[18];
+    const float qz = cf * c[7] + cr * c[11] + cu * c[15] - c[19];
+    const float lq = qt + (x * qx + y * qy + z * qz) / r;
+    const float pt = -qt + f * lq;
+    const float flr = f * lq / r;
+    const float px = qx + flr * x, py = qy + flr * y, pz = qz + flr * z;
+    const float scam = A.relativistic ? 1.0f / fminf(fmaxf(fabsf(pt), 0.05f), 20.0f) : 1.0f;
+    const float* sp = A.skyp;
+    const float cpsi = fminf(fmaxf(pt * sp[0] + px * sp[1] + py * sp[2] + pz * sp[3], -1.0f), 1.0f);
+    const float psi = acosf(cpsi);
+    const float tpsi = psi * ((float)A.N - 1.0f) / 3.14159265f;
+    float esc, ct, st;
+    fan_dir(A.fan, A.N, A.fine, A.Nf, sp[8], sp[9], psi, esc, ct, st);
+    float sky_r = 0.0f, sky_g = 0.0f, sky_b = 0.0f;
+    if (esc > 0.5f) {
+        // the asymptotic direction in this ray's own plane: e1 radial, e2 the
+        // unit lateral part of the coordinate velocity at the camera
+        const float inv_r = 1.0f / r;
+        const float k0 = (x * px + y * py + z * pz) * inv_r;
+        const float c10 = f * (-pt + k0) * inv_r;
+        const float vx = px - c10 * x, vy = py - c10 * y, vz = pz - c10 * z;
+        const float e1x = x * inv_r, e1y = y * inv_r, e1z = z * inv_r;
+        const float vr = vx * e1x + vy * e1y + vz * e1z;
+        float lx = vx - vr * e1x, ly = vy - vr * e1y, lz = vz - vr * e1z;
+        const float ll = sqrtf(lx * lx + ly * ly + lz * lz);
+        if (ll < 1e-7f) {
+            lx = -e1y; ly = e1x; lz = 0.0f;
+            const float lm = fmaxf(sqrtf(lx * lx + ly * ly), 1e-6f);
+            lx /= lm; ly /= lm;
+        } else {
+            lx /= ll; ly /= ll; lz /= ll;
+        }
+        const float dx = ct * e1x + st * lx, dy = ct * e1y + st * ly, dz = ct * e1z + st * lz;
+        const bool stars = D.star_params[0] > 0.0f;
+        // the texture is sampled unless the starfield replaces it (stars on
+        // and texture weight not positive: set_starfield!'s sky_tex false,
+        // src/metal.jl:293)
+        if (!stars || D.star_params[15] > 0.0f) {
+            const float thb = acosf(fminf(fmaxf(dz / fmaxf(sqrtf(dx * dx + dy * dy + dz * dz), 1e-9f), -1.0f), 1.0f));
+            const float phb = atan2f(dy, dx);
+            float col[3];
+            sample_sky<float>(D, thb, phb, col);
+            sky_r = col[0]; sky_g = col[1]; sky_b = col[2];
+        }
+        if (stars) {
+            const float dl = fmaxf(sqrtf(dx * dx + dy * dy + dz * dz), 1e-20f);
+            const float tw = D.star_params[15];
+            sky_r *= tw; sky_g *= tw; sky_b *= tw;
+            float stc[3];
+            starfield<float>(D, dx / dl, dy / dl, dz / dl, stc);
+            sky_r += D.star_params[0] * stc[0];
+            sky_g += D.star_params[0] * stc[1];
+            sky_b += D.star_params[0] * stc[2];
+        }
+        if (scam != 1.0f) {
+            sky_r *= 57.4f / (expf(4.067f / scam) - 1.0f);
+            sky_g *= 90.2f / (expf(4.513f / scam) - 1.0f);
+            sky_b *= 206.5f / (expf(5.335f / scam) - 1.0f);
+        }
+    }
+    float4 L = layer_tap(A.layer, i, j, A.width, A.height, A.lw, A.lh);
+    if (A.ring_rmin > 0.0f && A.ring) {
+        int rk0 = (int)tpsi;
+        rk0 = rk0 < 0 ? 0 : (rk0 > A.N - 2 ? A.N - 2 : rk0);
+        const float rfr = tpsi - (float)rk0;
+        const float rmin = A.fan[rk0].w + rfr * (A.fan[rk0 + 1].w - A.fan[rk0].w);
+        const float t = fminf(fmaxf((rmin - kRingFeather * A.ring_rmin) /
+                                        fmaxf((1.0f - kRingFeather) * A.ring_rmin, 1e-6f), 0.0f), 1.0f);
+        const float wr = 1.0f - t * t * (3.0f - 2.0f * t);
+        if (wr > 0.0f) {
+            const float4 R = layer_tap(A.ring, i, j, A.width, A.height, A.rw, A.rh);
+            L.x += wr * (R.x - L.x); L.y += wr * (R.y - L.y);
+            L.z += wr * (R.z - L.z); L.w += wr * (R.w - L.w);
+        }
+    }
+    float* o = A.out + ((size_t)j * A.width + i) * 3;
+    const float ore = L.x + L.w * sky_r, oge = L.y + L.w * sky_g, obe = L.z + L.w * sky_b;
+    if (A.accumulate) {
+        o[0] += ore; o[1] += oge; o[2] += obe;
+    } else {
+        o[0] = ore; o[1] = oge; o[2] = obe;
+    }
+}
+
+}  // namespace
+
+}  // namespace st
+
+// ---- C ABI ----
+using st::fail;
+
+struct st_image {
+    int device = 0;
+    int channels = 0, width = 0, height = 0;
+    float* buf = nullptr;
+};
+
+// What the fans depend on: the fan camera block, M, r_escape, dt, nmax (the
+// gas gate and the radial axis do not enter the integration).
+struct st_fan_key {
+    float cam[20];
+    float M, r_escape, dt;
+    int nmax;
+};
+
+struct st_skyfan {
+    int device = 0;
+    int n = 0, n_fine = 0;
+    float* fan = nullptr;    // float4 per entry
+    float* fine = nullptr;
+    float* skyp = nullptr;   // 12 floats
+    bool have_key = false;   // fans valid for `key`
+    st_fan_key key;
+    int recomputed = 0;      // 1 if the last update integrated the fans
+    bool speculate = true;   // fine fan beside the coarse one over a predicted band
+};
+
+extern "C" {
+
+int st_image_create(st_ctx* c, int channels, int w, int h, st_image** out) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if ((channels != 3 && channels != 4) || w < 2 || h < 2) return fail(ST_ERR_INVALID_ARG, "bad image shape");
+    if ((size_t)w * h > ((size_t)1 << 28)) return fail(ST_ERR_INVALID_ARG, "image too large");
+    ST_HIP(hipSetDevice(c->device));
+    st_image* m = new st_image();
+    m->device = c->device;
+    m->channels = channels; m->width = w; m->height = h;
+    const size_t bytes = sizeof(float) * channels * (size_t)w * h;
+    if (hipMalloc(&m->buf, bytes) != hipSuccess) {
+        delete m;
+        return fail(ST_ERR_OOM, "hipMalloc of the image failed");
+    }
+    if (hipMemset(m->buf, 0, bytes) != hipSuccess) {
+        (void)hipFree(m->buf);
+        delete m;
+        return fail(ST_ERR_DEVICE, "hipMemset of the image failed");
+    }
+    *out = m;
+    return ST_OK;
+}
+
+int st_image_destroy(st_image* m) {
+    if (!m) return ST_OK;
+    if (m->buf) {
+        (void)hipSetDevice(m->device);
+        (void)hipFree(m->buf);
+    }
+    delete m;
+    return ST_OK;
+}
+
+int st_image_download(st_ctx* c, const st_image* m, float* host, size_t n) {
+    if (!c || !m || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (n != (size_t)m->channels * m->width * m->height) return fail(ST_ERR_INVALID_ARG, "nfloats mismatch");
+    ST_HIP(hipMemcpyAsync(host, m->buf, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_image_upload(st_ctx* c, st_image* m, const float* host, size_t n) {
+    if (!c || !m || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (n != (size_t)m->channels * m->width * m->height) return fail(ST_ERR_INVALID_ARG, "nfloats mismatch");
+    ST_HIP(hipMemcpyAsync(m->buf, host, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_skyfan_create(st_ctx* c, int n, int n_fine, st_skyfan** out) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (n < 4 || n_fine < 4 || n > (1 << 24) || n_fine > (1 << 24)) return fail(ST_ERR_INVALID_ARG, "bad fan size");
+    ST_HIP(hipSetDevice(c->device));
+    st_skyfan* f = new st_skyfan();
+    f->device = c->device;
+    f->n = n; f->n_fine = n_fine;
+    if (hipMalloc(&f->fan, sizeof(float) * 4 * (size_t)n) != hipSuccess ||
+        hipMalloc(&f->fine, sizeof(float) * 4 * (size_t)n_fine) != hipSuccess ||
+        hipMalloc(&f->skyp, sizeof(float) * 12) != hipSuccess ||
+        hipMemset(f->fan, 0, sizeof(float) * 4 * (size_t)n) != hipSuccess ||
+        hipMemset(f->fine, 0, sizeof(float) * 4 * (size_t)n_fine) != hipSuccess ||
+        hipMemset(f->skyp, 0, sizeof(float) * 12) != hipSuccess) {
+        if (f->fan) (void)hipFree(f->fan);
+        if (f->fine) (void)hipFree(f->fine);
+        if (f->skyp) (void)hipFree(f->skyp);
+        delete f;
+        return fail(ST_ERR_OOM, "fan allocation failed");
+    }
+    *out = f;
+    return ST_OK;
+}
+
+int st_skyfan_destroy(st_skyfan* f) {
+    if (!f) return ST_OK;
+    (void)hipSetDevice(f->device);
+    if (f->fan) (void)hipFree(f->fan);
+    if (f->fine) (void)hipFree(f->fine);
+    if (f->skyp) (void)hipFree(f->skyp);
+    delete f;
+    return ST_OK;
+}
+
+int st_skyfan_update(st_ctx* c, st_skyfan* f, const float fan_cam[20], const float sky_params[12], int nmax,
+                     float dt) {
+    if (!c || !f || !fan_cam || !sky_params) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (!(sky_params[4] > 0.0f) || !(dt > 0.0f) || nmax < 1) return fail(ST_ERR_INVALID_ARG, "bad fan parameters");
+    st_fan_key key;
+    std::memset(&key, 0, sizeof(key));
+    std::memcpy(key.cam, fan_cam, sizeof(key.cam));
+    key.M = sky_params[4];
+    key.r_escape = sky_params[5];
+    key.dt = dt;
+    key.nmax = nmax;
+    hipStream_t q = c->stream;
+    ST_HIP(hipEventRecord(c->ev[0], q));
+    st::FanArgs A;
+    std::memset(&A, 0, sizeof(A));
+    std::memcpy(A.sky, sky_params, sizeof(float) * 8);   // band slots 8..11 zero
+    A.skyp = f->skyp;
+    if (f->have_key && std::memcmp(&key, &f->key, sizeof(key)) == 0) {
+        // The same fans (the integration is deterministic): refresh the radial
+        // axis, M, r_escape and the gate; the band stays.
+        hipLaunchKernelGGL(st::k_sky_head, dim3(1), dim3(64), 0, q, A);
+        f->recomputed = 0;
+    } else {
+        std::memcpy(A.cam, fan_cam, sizeof(A.cam));
+        A.M = key.M;
+        A.r_escape = key.r_escape;
+        A.dt = dt;
+        A.nmax = nmax;
+        A.fan = f->fan;
+        A.fine = f->fine;
+        A.N = f->n;
+        A.Nf = f->n_fine;
+        A.pad = 3;
+        A.write_sky = 1;
+        // predicted band: the coarse pair that straddles Synge's critical
+        // angle (capture beyond pi - asin(3 sqrt 3 (M/r) sqrt(1 - 2M/r)) from
+        // the outward axis at r >= 3M, beyond asin(...) inside), in the
+        // band kernel's own float arithmetic
+        const double M = key.M;
+        const double r = std::sqrt((double)fan_cam[0] * fan_cam[0] + (double)fan_cam[1] * fan_cam[1] +
+                                   (double)fan_cam[2] * fan_cam[2]);
+        if (!f->speculate) {
+            A.pred_lo = A.pred_hi = NAN;   // never equal: the second launch integrates the fine fan
+        } else if (r > 2.0 * M) {
+            const double sc = std::min(3.0 * std::sqrt(3.0) * M / r * std::sqrt(1.0 - 2.0 * M / r), 1.0);
+            const double psic = r >= 3.0 * M ? M_PI - std::asin(sc) : std::asin(sc);
+            int k = (int)std::floor(psic * (f->n - 1) / M_PI);
+            k = std::min(std::max(k, 0), f->n - 2);
+            const float dpsi = 3.14159265f / ((float)f->n - 1.0f);
+            A.pred_lo = std::fmax((float)k - (float)A.pad, 0.0f) * dpsi;
+            A.pred_hi = std::fmin((float)(k + 1) + (float)A.pad, (float)f->n - 1.0f) * dpsi;
+        }
+        A.nbc = (f->n + 255) / 256;
+        const int nbf = (f->n_fine + 255) / 256;
+        hipLaunchKernelGGL(st::k_sky_fans, dim3(A.nbc + (f->speculate ? nbf : 0)), dim3(256), 0, q, A);
+        hipLaunchKernelGGL(st::k_fan_band_fix, dim3(nbf), dim3(256), 0, q, A);
+        f->key = key;
+        f->have_key = true;
+        f->recomputed = 1;
+    }
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipEventRecord(c->ev[1], q));
+    ST_HIP(hipEventRecord(c->ev[2], q));
+    c->last_mixed = false;
+    c->have_last = true;   // timings of this call; the counters stay the last trace's
+    return ST_OK;
+}
+
+int st_skyfan_recomputed(const st_skyfan* f) { return f ? f->recomputed : 0; }
+
+int st_skyfan_set_speculate(st_skyfan* f, int on) {
+    if (!f) return fail(ST_ERR_INVALID_ARG, "null fan");
+    f->speculate = on != 0;
+    f->have_key = false;
+    return ST_OK;
+}
+
+int st_skyfan_download(st_ctx* c, const st_skyfan* f, float* fan, float* fine, float* sky_params) {
+    if (!c || !f) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (fan) ST_HIP(hipMemcpyAsync(fan, f->fan, sizeof(float) * 4 * (size_t)f->n, hipMemcpyDeviceToHost, c->stream));
+    if (fine)
+        ST_HIP(hipMemcpyAsync(fine, f->fine, sizeof(float) * 4 * (size_t)f->n_fine, hipMemcpyDeviceToHost, c->stream));
+    if (sky_params) ST_HIP(hipMemcpyAsync(sky_params, f->skyp, sizeof(float) * 12, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_trace_layer(st_ctx* c, const st_skyfan* f, st_image* layer, const float cam[49], const double* cam64,
+                   const float sp[8], const st_trace_args* a, float ring_rmin, int substride, int subx, int suby) {
+    if (!c || !f || !layer || !cam || !sp || !a) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (layer->channels != 4) return fail(ST_ERR_INVALID_ARG, "a layer has 4 channels");
+    if (a->width != layer->width || a->height != layer->height)
+        return fail(ST_ERR_INVALID_ARG, "args width/height must match the layer");
+    if (substride < 1 || subx < 0 || suby < 0 || subx >= substride || suby >= substride)
+        return fail(ST_ERR_INVALID_ARG, "bad sub-grid");
+    if (a->precision != ST_PREC_FP32 || a->order == 45 || a->order == 46 || a->stat)
+        return fail(ST_ERR_UNSUPPORTED, "the layer pass is FP32 fixed-step");
+    const int sw = (layer->width + substride - 1) / substride, sh = (layer->height + substride - 1) / substride;
+    const int cols = a->cols < 0 ? sw : a->cols;
+    if (a->row0 < 0 || a->rows < 0 || a->row0 + a->rows > sh || a->col0 < 0 || a->col0 + cols > sw)
+        return fail(ST_ERR_INVALID_ARG, "tile outside the dispatch");
+    st::TraceParams P;
+    int rc = st::fill_params(c, sp, a, P);
+    if (rc != ST_OK) return rc;
+    P.fan = f->fan;
+    P.fan_n = f->n;
+    P.sky_params = f->skyp;
+    P.ring_rmin = ring_rmin;
+    P.assign = 1;
+    st::sync_temperature_scale(c, sp[0]);
+    const st::DeviceScene* D = nullptr;
+    if ((rc = st::upload_scene(c, &D)) != ST_OK) return rc;
+    st::FrameRef F;
+    std::memset(&F, 0, sizeof(F));
+    std::memcpy(F.c32, cam, sizeof(F.c32));
+    if (cam64) { std::memcpy(F.c64, cam64, sizeof(F.c64)); F.has64 = 1; }
+    F.width = sw * substride;
+    F.height = sh * substride;
+    F.out = layer->buf;
+    st::TileGeom G;
+    G.col0 = a->col0; G.row0 = a->row0; G.cols = cols; G.rows = a->rows; G.tw = 8;
+    G.tiles_x = (cols + 7) / 8;
+    const int nblocks = G.tiles_x * ((a->rows + 31) / 32);
+    const bool kerr = sp[6] != 0.0f;
+    const bool vol = c->has_volume && c->vol_on;
+    hipStream_t q = c->stream;
+    ST_HIP(hipEventRecord(c->ev[0], q));
+    ST_HIP(st::launch_reset(c->qcount, c->counters, nullptr, q));
+    if (nblocks > 0) {
+#define ST_L(KK, VV) \
+    hipLaunchKernelGGL((st::k_trace_layer<KK, VV>), dim3(nblocks), dim3(256), 0, q, D, P, F, G, layer->width, \
+                       layer->height, substride, subx, suby)
+        if (kerr) { if (vol) ST_L(true, true); else ST_L(true, false); }
+        else { if (vol) ST_L(false, true); else ST_L(false, false); }
+#undef ST_L
+        ST_HIP(hipGetLastError());
+    }
+    ST_HIP(hipEventRecord(c->ev[1], q));
+    ST_HIP(hipEventRecord(c->ev[2], q));
+    c->last_mixed = false;
+    c->have_last = true;
+    c->last_counters = true;
+    return ST_OK;
+}
+
+int st_sky_composite(st_ctx* c, const st_skyfan* f, const st_image* layer, const st_image* ring, const float cam[49],
+                     const float sp[8], float ring_rmin, float ju, float jv, int accumulate, int row0, int rows) {
+    if (!c || !f || !layer || !cam || !sp) return fail(ST_ERR_INVALID_ARG, "null argument");
+    if (layer->channels != 4 || (ring && ring->channels != 4)) return fail(ST_ERR_INVALID_ARG, "layers have 4 channels");
+    if (rows < 0) rows = c->height - row0;
+    if (row0 < 0 || row0 + rows > c->height) return fail(ST_ERR_INVALID_ARG, "row band outside the frame");
+    const st::DeviceScene* D = nullptr;
+    int rc = st::upload_scene(c, &D);
+    if (rc != ST_OK) return rc;
+    st::CompositeArgs A;
+    std::memcpy(A.cam, cam, sizeof(A.cam));
+    A.M = sp[0];
+    A.relativistic = sp[3] > 0.5f;
+    A.fan = reinterpret_cast<const float4*>(f->fan);
+    A.fine = reinterpret_cast<const float4*>(f->fine);
+    A.N = f->n; A.Nf = f->n_fine;
+    A.skyp = f->skyp;
+    A.layer = reinterpret_cast<const float4*>(layer->buf);
+    A.ring = ring ? reinterpret_cast<const float4*>(ring->buf) : nullptr;
+    A.lw = layer->width; A.lh = layer->height;
+    A.rw = ring ? ring->width : layer->width;
+    A.rh = ring ? ring->height : layer->height;
+    A.ring_rmin = ring ? ring_rmin : 0.0f;
+    A.ju = ju; A.jv = jv;
+    A.accumulate = accumulate != 0;
+    A.width = c->width; A.height = c->height; A.row0 = row0; A.rows = rows;
+    A.out = c->out;
+    hipStream_t q = c->stream;
+    ST_HIP(hipEventRecord(c->ev[0], q));
+    const int n = c->width * rows;
+    if (n > 0) hipLaunchKernelGGL(st::k_sky_composite, dim3((n + 255) / 256), dim3(256), 0, q, D, A);
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipEventRecord(c->ev[1], q));
+    ST_HIP(hipEventRecord(c->ev[2], q));
+    c->last_mixed = false;
+    c->have_last = true;   // timings of this call; the counters stay the last trace's
+    return ST_OK;
+}
+
+}  // extern "C"

@@ -46,6 +46,13 @@ struct TraceParams {
     int bake;                // 1: warp-map texels (8 floats per pixel) instead of radiance
     float* stat;             // per-pixel luminance moments + graze weight (3 floats), or null
     int nb;                  // > 0: depth buckets, out holds 3 * nb floats per pixel
+    // layered engine (LAYER variant): the deflection fan (4 floats per entry:
+    // escaped, cos, sin, periapsis), its sky block (radial tetrad axis,
+    // M, r_escape, gas gate, ..., critical band) and the ring pass's radius
+    const float* fan;
+    int fan_n;
+    const float* sky_params;
+    float ring_rmin;
     float* out;
     float* rec;              // ST_REC_N floats per pixel, or null
     uint2* queue;            // promoted rays (frame, pixel)
@@ -550,7 +557,7 @@ __device__ __forceinline__ void bardeen(const S6<T>& s, T pt, T M, T a, bool& ca
 // promote_band, that the launch analysis finds captured, or that come within
 // promote_horizon * r_plus of the horizon are handed to the FP64 queue
 // (o.routed) instead of being finished here.
-template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT = false, bool DEPTH = false>
+template <typename T, bool KERR, bool VOL, bool ROUTE, bool ADAPT = false, bool DEPTH = false, bool LAYER = false>
 __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParams& P, const FrameRef& F, int i, int j,
                                           RayOut<T>& o) {
     using A = Ar<T>;
@@ -662,7 +669,35 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     // ---- launch-time radial potential (src/metal.jl:1008-1094) ----
     bool captured0 = false;
     const T rplus = M + A::psqrt(mx(M * M - a * a, T(0)));
-    const bool need_bardeen = !no_sky && ((KERR && P.bardeen && a != T(0)) || ROUTE);
+    // Layered mode (src/metal.jl:1095-1135): a pixel whose geodesic provably
+    // stays outside the gas gate — periapsis from the fan at its local angle
+    // psi to the radial tetrad axis — is transparent without integrating; the
+    // ring pass keeps only rays whose periapsis dips below ring_rmin.
+    T gate = T(0), r_layer_prev = T(-1);
+    if constexpr (LAYER) {
+        const float* sp7 = P.sky_params;
+        gate = T(sp7[6]);
+        if (r_cam > gate || P.ring_rmin > 0.0f) {
+            const T cpsi = clampv(pt * T(sp7[0]) + s.px * T(sp7[1]) + s.py * T(sp7[2]) + s.pz * T(sp7[3]), T(-1),
+                                  T(1));
+            const T Nf = T(P.fan_n);
+            const T tf = A::acos(cpsi) * (Nf - T(1)) / T(3.141592653589793);
+            int k0 = (int)tf;
+            k0 = k0 < 0 ? 0 : (k0 > P.fan_n - 2 ? P.fan_n - 2 : k0);
+            const T fr = tf - T(k0);
+            const float* f0 = P.fan + 4 * k0;
+            const T esc = T(f0[0]) + fr * (T(f0[4]) - T(f0[0]));
+            const T rmin = T(f0[3]) + fr * (T(f0[7]) - T(f0[3]));
+            const bool cull = P.ring_rmin > 0.0f ? rmin >= T(P.ring_rmin) : (esc > T(0.999) && rmin > gate);
+            if (cull) {
+                o.rgb[0] = o.rgb[1] = o.rgb[2] = T(0);
+                o.alpha = T(1);
+                o.fate = 0;
+                return;
+            }
+        }
+    }
+    const bool need_bardeen = !LAYER && !no_sky && ((KERR && P.bardeen && a != T(0)) || ROUTE);
     if (need_bardeen) {
         bool cap, graze, ing;
         T rc, Rc, Rpp;
@@ -747,6 +782,12 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
         const T r2 = s.x * s.x + s.y * s.y + s.z * s.z;
         const T r = (sizeof(T) == 4) ? r2 * A::rsq(r2) : A::sqrt(r2);
         if (P.rec) rk_min = mn(rk_min, rk2);
+        if constexpr (LAYER) {
+            // outbound beyond the gas gate: a null geodesic has at most one
+            // radial turning point, so it cannot re-enter (src/metal.jl:1202-1211)
+            if (r > gate && r_layer_prev > T(0) && r > r_layer_prev) break;
+            r_layer_prev = r;
+        }
         // the step's exit tests, evaluated together and taken in one branch
         // (capture first, then promotion, then escape)
         bool cap = false;
@@ -909,6 +950,14 @@ __device__ __forceinline__ void trace_ray(const DeviceScene& D, const TraceParam
     }
     o.steps = steps;
     o.gas_samples = nsamp;
+    if constexpr (LAYER) {
+        // the layer is the emission in front of the sky and the transmittance
+        // to it, whatever the fate (src/metal.jl:1613-1627)
+        o.rgb[0] = acc[0]; o.rgb[1] = acc[1]; o.rgb[2] = acc[2];
+        o.alpha = alpha;
+        o.fate = 0;
+        return;
+    }
     if constexpr (ROUTE) {
         if (routed) {
             o.routed = true;

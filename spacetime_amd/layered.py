@@ -1,0 +1,333 @@
+"""Layered real-time engine (src/metal.jl:2393-3142).
+
+By spherical symmetry the whole lensed sky seen by a static observer at
+radius r is a function of one local angle psi between the ray and the
+outward radial axis.  Each frame integrates an exact fan of geodesics over
+psi in [0, pi] (a few thousand rays, plus a refinement fan across the
+escape/capture transition), and every display pixel becomes a table lookup
+and a sky sample.  Only the disc/gas, which breaks the symmetry, is traced
+per pixel, as a premultiplied layer (its own resolution; rays the fan proves
+empty are culled) that the composite lays over the sky.  An optional ring
+pass traces the strongly wound rays (fan periapsis below ``ring_rmin``) at a
+finer resolution, restricted to a screen box that contains them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .api import (Camera, HipPreviewContext, StepRule, camera_beta, ks_cam_params, ks_camera_tetrad, spin,
+                  spacetime_params, trace_args)
+
+RING_FEATHER = 0.8     # src/metal.jl:2532
+RING_B_MARGIN = 1.06   # src/metal.jl:2540
+
+
+class Image:
+    """A (channels, w, h) float image in device memory (st_image); a layer
+    has 4 channels: premultiplied RGB and transmittance to the sky."""
+
+    def __init__(self, ctx: HipPreviewContext, width: int, height: int, channels: int = 4):
+        self.ctx = ctx
+        self.width, self.height, self.channels = int(width), int(height), int(channels)
+        h = C.c_void_p()
+        L.check(L.lib().st_image_create(ctx.handle, self.channels, self.width, self.height, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def download(self) -> np.ndarray:
+        """float32 (h, w, channels)."""
+        out = np.empty((self.height, self.width, self.channels), dtype=np.float32)
+        L.check(L.lib().st_image_download(self.ctx.handle, self._h, L.fptr(out), out.size))
+        return out
+
+    def upload(self, pixels: np.ndarray):
+        p = np.ascontiguousarray(pixels, dtype=np.float32)
+        if p.shape != (self.height, self.width, self.channels):
+            raise ValueError(f"pixels must be ({self.height}, {self.width}, {self.channels})")
+        L.check(L.lib().st_image_upload(self.ctx.handle, self._h, L.fptr(p), p.size))
+
+    def fill_transparent(self):
+        """RGB 0, transmittance 1: the frame is the fan-driven sky alone."""
+        p = np.zeros((self.height, self.width, self.channels), dtype=np.float32)
+        p[..., 3] = 1.0
+        self.upload(p)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().st_image_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SkyFanState:
+    """SkyFanState (src/metal.jl:2957-2968): the coarse fan (uniform in psi
+    over [0, pi]), the refinement fan across the critical band and the sky
+    parameter block, in device memory."""
+
+    def __init__(self, ctx: HipPreviewContext, n: int = 4096, n_fine: int = 1024):
+        self.ctx = ctx
+        self.n, self.n_fine = int(n), int(n_fine)
+        h = C.c_void_p()
+        L.check(L.lib().st_skyfan_create(ctx.handle, self.n, self.n_fine, C.byref(h)))
+        self._h = h
+        self.gate = 0.0
+
+    @property
+    def handle(self):
+        return self._h
+
+    def download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(coarse (n, 4), fine (n_fine, 4), sky_params (12,)); an entry is
+        (escaped, cos thf, sin thf, r_min)."""
+        fan = np.empty((self.n, 4), dtype=np.float32)
+        fine = np.empty((self.n_fine, 4), dtype=np.float32)
+        skp = np.empty(12, dtype=np.float32)
+        L.check(L.lib().st_skyfan_download(self.ctx.handle, self._h, L.fptr(fan), L.fptr(fine), L.fptr(skp)))
+        return fan, fine, skp
+
+    @property
+    def recomputed(self) -> bool:
+        """True if the last update integrated the fans (False: reused, the
+        radius and step rule unchanged)."""
+        return bool(L.lib().st_skyfan_recomputed(self._h))
+
+    def set_speculate(self, on: bool):
+        """Integrate the fine fan beside the coarse one over a predicted band
+        (default), or in sequence; the fans are the same either way."""
+        L.check(L.lib().st_skyfan_set_speculate(self._h, int(bool(on))))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().st_skyfan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def layered_nmax(ctx: HipPreviewContext, r_escape: float, M: float, dt: float) -> int:
+    """The step budget of update_sky_fan! and render_layered_gpu!
+    (src/metal.jl:3000-3002, 3056-3058)."""
+    return int(min(max(ctx.nmax, math.ceil((75.0 + 6.5 * math.log(r_escape / M)) * M / dt)), 20000))
+
+
+def fan_camera_block(r: float, M: float) -> np.ndarray:
+    """The fan camera's block (src/metal.jl:2984-2991): the static observer
+    at (r, 0, 0) with fwd = x (outward radial) and right = y (the fan's
+    lateral axis): position, 1, Ef, Er, Eu, u."""
+    u, ef, er, eu = ks_camera_tetrad((r, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), M)
+    return np.array([r, 0.0, 0.0, 1.0, *ef, *er, *eu, *u], dtype=np.float32)
+
+
+def sky_params_block(pos, M: float, r_escape: float, gate: float) -> np.ndarray:
+    """sky_params (src/metal.jl:2992-2999): the radial tetrad axis at `pos`
+    (the forward leg of the static tetrad pointing along pos, any completion
+    of the basis), M, r_escape, the gas gate radius; the band kernel fills
+    slots 8-9."""
+    pos = np.asarray(pos, dtype=np.float64)
+    xh = pos / np.linalg.norm(pos)
+    a = np.array([0.0, 0.0, 1.0]) if abs(xh[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    b1 = np.cross(a, xh)
+    b1 /= np.linalg.norm(b1)
+    b2 = np.cross(xh, b1)
+    er = ks_camera_tetrad(pos, xh, b1, b2, M)[1]
+    return np.array([*er, M, r_escape, gate, 0.0, 0.0, 0.0, 0.0, 0.0], dtype=np.float32)
+
+
+def update_sky_fan(sky: SkyFanState, ctx: HipPreviewContext, pos, spacetime, *, gate: float,
+                   dt: Optional[float] = None):
+    """update_sky_fan! (src/metal.jl:2978-3028): rebuild the fan for a
+    camera at `pos` (radius |pos|), the band and the fine fan, on the
+    context's stream."""
+    dt = ctx.dt if dt is None else float(dt)
+    M = float(spacetime.M)
+    r = float(np.linalg.norm(pos))
+    r_escape = ctx.r_escape_factor * max(r, 15.0 * M)
+    cam = fan_camera_block(r, M)
+    skp = sky_params_block(pos, M, r_escape, gate)
+    nmax = layered_nmax(ctx, r_escape, M, dt)
+    L.check(L.lib().st_skyfan_update(ctx.handle, sky.handle, L.fptr(cam), L.fptr(skp), nmax, dt))
+    sky.gate = float(gate)
+
+
+def wound_impact_parameter(M: float, ring_rmin: float) -> float:
+    """_wound_impact_parameter (src/metal.jl:2555-2559): the impact parameter
+    b(R) = R / sqrt(1 - 2M/R) below which a null geodesic's periapsis falls
+    under R (b is increasing in the periapsis above 3M and captured rays lie
+    below b_c = 3 sqrt(3) M); infinite when R <= 2M."""
+    R = float(ring_rmin)
+    if not R > 2.0 * M:
+        return math.inf
+    return R / math.sqrt(1.0 - 2.0 * M / R)
+
+
+def ring_screen_box(cam: Camera, M: float, ring_rmin: float, fisheye_deg: float, rw: int, rh: int, *,
+                    probes: int = 81, margin: int = 3) -> Optional[tuple[int, int, int, int]]:
+    """_ring_screen_box (src/metal.jl:2582-2666): the screen box, in the ring
+    buffer's rw x rh grid, of the rays with b < b(ring_rmin) * 1.06, found
+    by building the real pixel rays (tetrad, lens mapping, index lowering)
+    on a coarse probe grid; (col0, cols, row0, rows) or None for the whole
+    buffer.  b = L / E is conserved, so one evaluation at the camera decides
+    the whole geodesic."""
+    bR = wound_impact_parameter(M, ring_rmin) * RING_B_MARGIN
+    if not math.isfinite(bR):
+        return None
+    Mf = float(M)
+    u4, Ef, Er, Eu = ks_camera_tetrad(cam.pos, cam.fwd, cam.right, cam.up_local, Mf, beta=camera_beta(cam))
+    x, y, z = (float(v) for v in cam.pos)
+    r = math.sqrt(x * x + y * y + z * z)
+    if not r > 2.0 * Mf:
+        return None
+    f = 2.0 * Mf / r
+    nv = max(9, probes)
+    nu = max(9, round(probes * rw / rh))
+    half_h = rh / 2.0
+    # probe pixel centres, 1-based, and their (u, v) at the kernel's ju = jv = 0.5
+    ipx = 1.0 + np.arange(nu) * (rw - 1) / (nu - 1)
+    jpx = 1.0 + np.arange(nv) * (rh - 1) / (nv - 1)
+    I, J = np.meshgrid(ipx, jpx)          # (nv, nu), row-major in j then i
+    u = (I - 1.0 + 0.5 - rw / 2.0) / half_h
+    v = (J - 1.0 + 0.5 - rh / 2.0) / half_h
+    if fisheye_deg > 0:
+        rho = np.sqrt(u * u + v * v)
+        tp = rho * math.radians(max(float(fisheye_deg), 0.0))
+        st_ = np.sin(tp)
+        inv = np.where(rho > 1e-8, 1.0 / np.maximum(rho, 1e-300), 0.0)
+        cr, cu, cf = st_ * u * inv, st_ * v * inv, np.cos(tp)
+    else:
+        fov = float(cam.fov_factor)
+        dxl, dyl = u * fov, v * fov
+        nu_ = np.sqrt(dxl * dxl + dyl * dyl + 1.0)
+        cr, cu, cf = dxl / nu_, dyl / nu_, 1.0 / nu_
+    q = [cf * Ef[k] + cr * Er[k] + cu * Eu[k] - u4[k] for k in range(4)]
+    lq = q[0] + (x * q[1] + y * q[2] + z * q[3]) / r
+    pt = -q[0] + f * lq
+    flr = f * lq / r
+    px, py, pz = q[1] + flr * x, q[2] + flr * y, q[3] + flr * z
+    E = np.abs(pt)
+    ok = E > 1e-12
+    Lx, Ly, Lz = y * pz - z * py, z * px - x * pz, x * py - y * px
+    b = np.where(ok, np.sqrt(Lx * Lx + Ly * Ly + Lz * Lz) / np.where(ok, E, 1.0), np.inf)
+    # Julia's round (ties to even) of the probe centres
+    ip = np.round(I).astype(np.int64)
+    jp = np.round(J).astype(np.int64)
+    inside = ok & (b < bR)
+    if inside.any():
+        imin, imax = int(ip[inside].min()), int(ip[inside].max())
+        jmin, jmax = int(jp[inside].min()), int(jp[inside].max())
+    else:
+        # a wound region smaller than the probe spacing: seed at the probe of
+        # least b (the first in column-fastest order), pad by two spacings
+        k = int(np.argmin(b))
+        imin = imax = int(ip.flat[k])
+        jmin = jmax = int(jp.flat[k])
+    du = 2 * math.ceil((rw - 1) / (nu - 1)) + margin
+    dv = 2 * math.ceil((rh - 1) / (nv - 1)) + margin
+    c0 = min(max(imin - du, 1), rw) - 1
+    c1 = min(max(imax + du, 1), rw)
+    r0 = min(max(jmin - dv, 1), rh) - 1
+    r1 = min(max(jmax + dv, 1), rh)
+    cols, rows = c1 - c0, r1 - r0
+    if cols * rows > 0.55 * rw * rh:
+        return None
+    return (c0, cols, r0, rows)
+
+
+def _frame_setup(ctx: HipPreviewContext, cam: Camera, spacetime, fisheye_deg: float, relativistic: bool,
+                 dt: Optional[float], order: int):
+    dt = ctx.dt if dt is None else float(dt)
+    M = float(spacetime.M)
+    rule = StepRule(dt=dt, r_escape_factor=ctx.r_escape_factor, order=order)
+    r_escape = rule.escape_radius(cam, M)
+    rule.nmax = layered_nmax(ctx, r_escape, M, dt)
+    c32, c64 = ks_cam_params(cam, M, spin(spacetime), fisheye_deg=fisheye_deg)
+    sp = spacetime_params(spacetime, cam, rule, relativistic)
+    return rule, c32, c64, sp
+
+
+def layer_pass(ctx: HipPreviewContext, sky: SkyFanState, cam: Camera, spacetime, layer: Image, *,
+               fisheye_deg: float = 0.0, relativistic: bool = False, dt: Optional[float] = None,
+               substride: int = 1, subx: int = 0, suby: int = 0, ju: float = 0.5, jv: float = 0.5,
+               ring_rmin: float = 0.0, row0: int = 0, rows: Optional[int] = None, col0: int = 0,
+               cols: int = -1, order: int = 4):
+    """One LAYER dispatch (render_layered_gpu!'s layer or ring pass,
+    src/metal.jl:3079-3113): rows/cols in dispatch pixels
+    (cld(w, substride) x cld(h, substride)); asynchronous."""
+    if layer.channels != 4:
+        raise ValueError("a layer has 4 channels")
+    rule, c32, c64, sp = _frame_setup(ctx, cam, spacetime, fisheye_deg, relativistic, dt, order)
+    sh = -(-layer.height // substride)
+    a = trace_args(layer.width, layer.height, rule, ju=ju, jv=jv, row0=row0,
+                   rows=(sh - row0 if rows is None else rows), col0=col0, cols=cols)
+    L.check(L.lib().st_trace_layer(ctx.handle, sky.handle, layer.handle, L.fptr(c32), L.dptr(c64), L.fptr(sp),
+                                   C.byref(a), float(ring_rmin), int(substride), int(subx), int(suby)))
+
+
+def composite(ctx: HipPreviewContext, sky: SkyFanState, cam: Camera, spacetime, layer: Image, *,
+              ring: Optional[Image] = None, ring_rmin: float = 0.0, fisheye_deg: float = 0.0,
+              relativistic: bool = False, ju: float = 0.5, jv: float = 0.5, accumulate: bool = False,
+              row0: int = 0, rows: int = -1):
+    """sky_composite_kernel! into the context's (3, W, H) buffer
+    (src/metal.jl:3116-3141); asynchronous."""
+    if layer.channels != 4 or (ring is not None and ring.channels != 4):
+        raise ValueError("layers have 4 channels")
+    rule, c32, c64, sp = _frame_setup(ctx, cam, spacetime, fisheye_deg, relativistic, None, 4)
+    use_ring = ring is not None and ring_rmin > 0
+    L.check(L.lib().st_sky_composite(ctx.handle, sky.handle, layer.handle, ring.handle if use_ring else None,
+                                     L.fptr(c32), L.fptr(sp), float(ring_rmin) if use_ring else 0.0, float(ju),
+                                     float(jv), int(bool(accumulate)), int(row0), int(rows)))
+
+
+def render_layered(ctx: HipPreviewContext, sky: SkyFanState, cam: Camera, spacetime, layer: Image, *,
+                   fisheye_deg: float = 0.0, relativistic: bool = False, dt: Optional[float] = None,
+                   trace_layer: bool = True, substride: int = 1, subx: int = 0, suby: int = 0,
+                   ju: float = 0.5, jv: float = 0.5, accumulate: bool = False, row0: int = 0, rows: int = -1,
+                   ring: Optional[Image] = None, ring_rmin: float = 0.0, ring_box: bool = True,
+                   order: int = 4):
+    """render_layered_gpu! (src/metal.jl:3040-3142): the disc/gas layer
+    (gate-culled by the fan), the optional ring pass, then the composite into
+    the context's (3, W, H) buffer, all asynchronous on the context's stream.
+    Call update_sky_fan for the camera's position first.  With
+    trace_layer=False the layer keeps what it holds (fill_transparent() for
+    the sky alone)."""
+    if layer.channels != 4 or (ring is not None and ring.channels != 4):
+        raise ValueError("layers have 4 channels")
+    width, height = ctx.width, ctx.height
+    if rows < 0:
+        rows = height
+    banded = not (row0 == 0 and rows == height)
+    if banded and ((layer.width, layer.height) != (width, height) or substride != 1):
+        raise ValueError("row-banded layered render needs a display-sized layer and substride 1")
+    kw = dict(fisheye_deg=fisheye_deg, relativistic=relativistic, dt=dt, ju=ju, jv=jv, order=order)
+    if trace_layer:
+        sw = -(-layer.width // substride)
+        sh = -(-layer.height // substride)
+        lrow0, lrows = (row0, rows) if banded else (0, sh)
+        layer_pass(ctx, sky, cam, spacetime, layer, substride=substride, subx=subx, suby=suby, row0=lrow0,
+                   rows=lrows, col0=0, cols=sw, **kw)
+    use_ring = ring is not None and ring_rmin > 0 and trace_layer and not banded
+    if use_ring:
+        rw, rh = ring.width, ring.height
+        box = ring_screen_box(cam, float(spacetime.M), ring_rmin, fisheye_deg, rw, rh) if ring_box else None
+        rc0, rcn, rr0, rrn = (0, rw, 0, rh) if box is None else box
+        layer_pass(ctx, sky, cam, spacetime, ring, ring_rmin=ring_rmin, row0=rr0, rows=rrn, col0=rc0, cols=rcn,
+                   **kw)
+    composite(ctx, sky, cam, spacetime, layer, ring=ring if use_ring else None, ring_rmin=ring_rmin,
+              fisheye_deg=fisheye_deg, relativistic=relativistic, ju=ju, jv=jv, accumulate=accumulate,
+              row0=row0, rows=rows)

@@ -1,0 +1,363 @@
+"""The layered engine on the GPU (src/metal.jl:2393-3142) against the FP64
+oracle: the sky fan and its refinement band, the composite on the same fan,
+the gate-culled disc layer, and the engine's own invariants (a sky-only frame
+reproduces the direct trace, sub-grid passes assemble the full layer, the
+ring box contains every wound ray, row bands assemble the full frame)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from spacetime_amd import _lib as L
+from spacetime_amd import api, layered, scenes
+
+pytestmark = pytest.mark.gpu
+
+import oracle  # noqa: E402
+
+M = 1.0
+DT = 0.02
+SCH = api.Schwarzschild(M)
+
+
+def smooth_sky(h=256, w=512):
+    th = (np.arange(h) + 0.5) / h * math.pi
+    ph = (np.arange(w) + 0.5) / w * 2 * math.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    sky = np.stack([0.5 + 0.3 * np.sin(T) * np.cos(P), 0.5 + 0.25 * np.cos(T), 0.4 + 0.2 * np.sin(T) * np.sin(2 * P)],
+                   axis=-1)
+    return sky.astype(np.float32)
+
+
+def context(W, H, disc=None, sky=None):
+    return api.HipPreviewContext(smooth_sky() if sky is None else sky, W, H, dt=DT, nmax=1000, disc=disc)
+
+
+def camera(pos=(0.0, -30.0, 3.0), fov=None):
+    return api.Camera(pos, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0) if fov is None else fov)
+
+
+def r_escape(ctx, pos):
+    return ctx.r_escape_factor * max(float(np.linalg.norm(pos)), 15.0 * M)
+
+
+def oracle_fan(ctx, pos, n, gate, skp_band=None, nf=None):
+    r = float(np.linalg.norm(pos))
+    cam20 = layered.fan_camera_block(r, M).astype(np.float64)
+    skp = layered.sky_params_block(pos, M, r_escape(ctx, pos), gate).astype(np.float64)
+    nmax = layered.layered_nmax(ctx, r_escape(ctx, pos), M, DT)
+    fan, _ = oracle.sky_fan(cam20, skp, nmax, DT, n)
+    fine = None
+    if skp_band is not None:
+        skp[8:10] = skp_band
+        fine, _ = oracle.sky_fan(cam20, skp, nmax, DT, nf, band=True)
+    return fan, fine, skp
+
+
+def oracle_frame_scene(ctx, cam, W, H, disc=None, sky=None, stars=None, relativistic=False):
+    c32, c64 = api.ks_cam_params(cam, M, 0.0)
+    dd = None
+    if disc is not None:
+        bb = disc.blackbody
+        dd = dict(inner=disc.inner_radius, outer=disc.outer_radius, falloff=disc.density_falloff,
+                  tmin=bb.table_min, tmax=bb.table_max,
+                  lut=oracle.blackbody_lut(bb.wb_temperature, bb.table_min, bb.table_max, bb.table_size),
+                  T0=disc.T0, r0=disc.r0, hold=disc.hold_isco)
+    r_esc = float(np.float32(r_escape(ctx, cam.pos)))
+    return oracle.Scene(M=M, a=0.0, cam_pos=cam.pos, cam_target=cam.target, cam_up=cam.up, fov=cam.fov_factor,
+                        width=W, height=H, r_escape=r_esc, relativistic=relativistic, dt=DT, h_coef=0.16,
+                        h_floor=1.0, h_cap=8.0, kerr_shrink=True, capture_mode=0,
+                        nmax=layered.layered_nmax(ctx, r_esc, M, DT), esc_interp=False, bardeen=True, disc=dd,
+                        sky=smooth_sky() if sky is None else sky, stars=stars, cam=c64)
+
+
+@pytest.mark.parametrize("pos", [(0.0, -30.0, 3.0), (9.0, 4.0, -2.0)])
+def test_fan_matches_oracle(hip, pos):
+    n, nf = 1025, 257
+    print(f"case: sky fan n={n} nf={nf} at {pos}", flush=True)
+    ctx = context(8, 8)
+    sky = layered.SkyFanState(ctx, n=n, n_fine=nf)
+    layered.update_sky_fan(sky, ctx, pos, SCH, gate=22.0)
+    fan, fine, skp = sky.download()
+    sky.close()
+    ctx2 = context(8, 8)
+    ofan, ofine, oskp = oracle_fan(ctx2, pos, n, 22.0, skp_band=skp[8:10].astype(np.float64), nf=nf)
+    ctx.close()
+    ctx2.close()
+    r0 = float(np.linalg.norm(pos))
+    pc = math.pi - math.asin(3 * math.sqrt(3) * M / r0 * math.sqrt(1 - 2 * M / r0))
+    psi = np.pi * np.arange(n) / (n - 1)
+    # band: the oracle's rule on the GPU's own coarse fan
+    band = oracle.fan_band(fan.astype(np.float64), oskp, 3)
+    assert skp[8] == pytest.approx(band[8], abs=1e-6) and skp[9] == pytest.approx(band[9], abs=1e-6)
+    assert skp[8] <= pc <= skp[9]
+    np.testing.assert_allclose(skp[:8], oskp[:8], rtol=1e-6, atol=1e-7)
+    # escape flags: equal away from the transition
+    far = np.abs(psi - pc) > 3 * np.pi / (n - 1)
+    assert np.array_equal(fan[far, 0], ofan[far, 0])
+    esc = far & (ofan[:, 0] == 1) & (np.abs(psi - pc) > 0.05)
+    ang = np.abs(np.arctan2(fan[esc, 2] * ofan[esc, 1] - fan[esc, 1] * ofan[esc, 2],
+                            fan[esc, 1] * ofan[esc, 1] + fan[esc, 2] * ofan[esc, 2]))
+    rrel = np.abs(fan[:, 3] - ofan[:, 3]) / ofan[:, 3]
+    fpsi = skp[8] + (skp[9] - skp[8]) * np.arange(nf) / (nf - 1)
+    ffar = np.abs(fpsi - pc) > 0.1 * (skp[9] - skp[8])
+    fesc = ffar & (ofine[:, 0] == 1)
+    fang = np.abs(np.arctan2(fine[fesc, 2] * ofine[fesc, 1] - fine[fesc, 1] * ofine[fesc, 2],
+                             fine[fesc, 1] * ofine[fesc, 1] + fine[fesc, 2] * ofine[fesc, 2]))
+    print(f"coarse dir max {ang.max():.2e} p50 {np.median(ang):.2e}; r_min rel max {rrel.max():.2e}; "
+          f"fine flags equal {np.mean(fine[ffar, 0] == ofine[ffar, 0]):.4f} dir max {fang.max():.2e}")
+    assert ang.max() < 1e-4 and np.median(ang) < 1e-6
+    assert rrel.max() < 5e-5
+    assert np.array_equal(fine[ffar, 0], ofine[ffar, 0])
+    assert fang.max() < 2e-3
+
+
+def test_fan_reuse_and_speculation_leave_the_fans_unchanged(hip):
+    """An update at the same radius reuses the fans (the sky block's radial
+    axis alone changes); the speculative fine fan equals the sequential one
+    bit for bit, at a radius outside and inside the photon sphere."""
+    print("case: fan reuse and speculation", flush=True)
+    ctx = context(8, 8)
+    sky = layered.SkyFanState(ctx, n=1025, n_fine=257)
+    a = np.array([0.0, -30.0, 3.0])
+    rot = np.array([[0.6, -0.8, 0.0], [0.8, 0.6, 0.0], [0.0, 0.0, 1.0]])
+    b = rot @ a
+    layered.update_sky_fan(sky, ctx, a, SCH, gate=22.0)
+    assert sky.recomputed
+    fa, fia, sa = sky.download()
+    layered.update_sky_fan(sky, ctx, b, SCH, gate=21.0)
+    assert not sky.recomputed
+    fb, fib, sb = sky.download()
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(fia, fib)
+    np.testing.assert_array_equal(sa[8:10], sb[8:10])
+    want = layered.sky_params_block(b, M, r_escape(ctx, b), 21.0)
+    np.testing.assert_array_equal(sb[:8], want[:8])
+    for pos in (a, np.array([2.4, 1.1, -0.5])):
+        layered.update_sky_fan(sky, ctx, pos, SCH, gate=22.0)
+        assert sky.recomputed or pos is a
+        f1, fi1, s1 = sky.download()
+        sky.set_speculate(False)
+        layered.update_sky_fan(sky, ctx, pos, SCH, gate=22.0)
+        assert sky.recomputed
+        f2, fi2, s2 = sky.download()
+        sky.set_speculate(True)
+        np.testing.assert_array_equal(f1, f2)
+        np.testing.assert_array_equal(fi1, fi2)
+        np.testing.assert_array_equal(s1, s2)
+        assert np.count_nonzero(np.diff(f1[:, 0])) >= 1
+    sky.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("stars,relativistic", [(False, False), (True, True)])
+def test_composite_matches_oracle_on_the_same_fan(hip, stars, relativistic):
+    W, H = 160, 96
+    print(f"case: composite {W}x{H} stars={stars} relativistic={relativistic}", flush=True)
+    ctx = context(W, H)
+    sp = ctx.set_starfield(height=H) if stars else None
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=2049, n_fine=513)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    rng = np.random.default_rng(11)
+    lay = np.empty((H // 2, W // 2, 4), np.float32)
+    lay[..., :3] = rng.uniform(0.0, 0.3, lay.shape[:2] + (3,))
+    lay[..., 3] = rng.uniform(0.2, 1.0, lay.shape[:2])
+    layer = layered.Image(ctx, W // 2, H // 2)
+    layer.upload(lay)
+    layered.render_layered(ctx, sky, cam, SCH, layer, trace_layer=False, relativistic=relativistic)
+    got = ctx.download().astype(np.float64)
+    fan, fine, skp = sky.download()
+    for o in (layer, sky):
+        o.close()
+    ctx.close()
+    st = None
+    if stars:
+        st = (sp, oracle.blackbody_lut(10000.0, float(sp[11]), float(sp[12]), int(sp[13])))
+    o = oracle_frame_scene(ctx, cam, W, H, stars=st, relativistic=relativistic)
+    want = o.sky_composite(fan.astype(np.float64), fine.astype(np.float64), skp.astype(np.float64), lay)
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-2)
+    print(f"rel diff p50 {np.median(rel):.2e} p99.9 {np.percentile(rel, 99.9):.2e} max {rel.max():.2e}")
+    assert np.all(np.isfinite(got))
+    assert np.median(rel) < 1e-5 and np.percentile(rel, 99.9) < 1e-3
+
+
+def test_layer_matches_oracle(hip):
+    W, H = 96, 54
+    disc = scenes.config_disc()
+    print(f"case: disc layer {W}x{H} gate 22", flush=True)
+    ctx = context(W, H, disc=disc)
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=2049, n_fine=513)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    layer = layered.Image(ctx, W, H)
+    layered.render_layered(ctx, sky, cam, SCH, layer)
+    st = ctx.stats()           # the composite keeps the layer trace's counters
+    got = layer.download().astype(np.float64)
+    fan, fine, skp = sky.download()
+    for o in (layer, sky):
+        o.close()
+    ctx.close()
+    o = oracle_frame_scene(ctx, cam, W, H, disc=disc)
+    fan64, skp64 = fan.astype(np.float64), skp.astype(np.float64)
+    want = np.array([[o.trace_layer(i, j, fan64, skp64) for i in range(W)] for j in range(H)])
+    culled_g = (got[..., 3] == 1.0) & np.all(got[..., :3] == 0.0, axis=-1)
+    culled_o = (want[..., 3] == 1.0) & np.all(want[..., :3] == 0.0, axis=-1)
+    agree = np.mean(culled_g == culled_o)
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-2)
+    print(f"culled {culled_o.mean():.3f} agreement {agree:.5f}; rel p50 {np.median(rel):.2e} "
+          f"p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e}; rays {st['rays']} steps {st['steps_fp32']}")
+    assert st["rays"] == W * H
+    assert 0.05 < culled_o.mean() < 0.95 and agree > 0.998
+    assert (got[..., :3] > 0).any()
+    assert np.median(rel) < 1e-5 and np.percentile(rel, 99) < 1e-3
+
+
+def test_sky_only_frame_reproduces_the_direct_trace(hip):
+    W, H = 256, 144
+    print(f"case: sky-only layered vs direct {W}x{H}", flush=True)
+    ctx = context(W, H)
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=4096, n_fine=1024)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    layer = layered.Image(ctx, W // 2, H // 2)
+    layer.fill_transparent()
+    layered.render_layered(ctx, sky, cam, SCH, layer, trace_layer=False)
+    comp = ctx.download()
+    direct = api.render_preview_hip(ctx, cam, SCH)
+    for o in (layer, sky):
+        o.close()
+    ctx.close()
+    dark_c, dark_d = comp.sum(-1) < 1e-6, direct.sum(-1) < 1e-6
+    agree = np.mean(dark_c == dark_d)
+    both = ~dark_c & ~dark_d
+    rel = (np.abs(comp - direct).sum(-1) / direct.sum(-1))[both]
+    print(f"fate agreement {agree:.5f}; rel p50 {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    assert agree > 0.995 and dark_d.any()
+    assert np.median(rel) < 1e-4 and np.percentile(rel, 99) < 1e-2
+
+
+def test_subgrid_passes_assemble_the_full_layer(hip):
+    W, H = 64, 36
+    print("case: substride 2 assembly 64x36", flush=True)
+    disc = scenes.config_disc()
+    ctx = context(W, H, disc=disc)
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=1025, n_fine=257)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    full = layered.Image(ctx, W, H)
+    layered.render_layered(ctx, sky, cam, SCH, full)
+    parts = layered.Image(ctx, W, H)
+    seen = []
+    for sy in range(2):
+        for sx in range(2):
+            layered.render_layered(ctx, sky, cam, SCH, parts, substride=2, subx=sx, suby=sy)
+            seen.append(parts.download())
+    a, b = full.download(), parts.download()
+    for o in (full, parts, sky):
+        o.close()
+    ctx.close()
+    np.testing.assert_array_equal(a, b)
+    # each pass wrote only its own sub-grid
+    assert np.all(seen[0][1::2, :] == 0.0) and np.all(seen[0][:, 1::2] == 0.0)
+    np.testing.assert_array_equal(seen[0][0::2, 0::2], a[0::2, 0::2])
+
+
+def test_ring_box_contains_every_wound_ray(hip):
+    W, H = 192, 108
+    rw, rh = 160, 90
+    ring_rmin = 6.0
+    print(f"case: ring pass {rw}x{rh} ring_rmin {ring_rmin}", flush=True)
+    disc = scenes.config_disc()
+    ctx = context(W, H, disc=disc)
+    cam = camera(fov=api.Lens(50.0))
+    box = layered.ring_screen_box(cam, M, ring_rmin, 0.0, rw, rh)
+    assert box is not None
+    c0, cn, r0, rn = box
+    sky = layered.SkyFanState(ctx, n=2049, n_fine=513)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    layer = layered.Image(ctx, W // 2, H // 2)
+    ring_a = layered.Image(ctx, rw, rh)
+    ring_b = layered.Image(ctx, rw, rh)
+    layered.render_layered(ctx, sky, cam, SCH, layer, ring=ring_a, ring_rmin=ring_rmin, ring_box=True)
+    with_ring = ctx.download()
+    layered.render_layered(ctx, sky, cam, SCH, layer, ring=ring_b, ring_rmin=ring_rmin, ring_box=False)
+    layered.render_layered(ctx, sky, cam, SCH, layer)
+    no_ring = ctx.download()
+    a, b = ring_a.download(), ring_b.download()
+    fan, _, _ = sky.download()
+    for o in (layer, ring_a, ring_b, sky):
+        o.close()
+    ctx.close()
+    inside = np.zeros((rh, rw), bool)
+    inside[r0:r0 + rn, c0:c0 + cn] = True
+    print(f"box {box}: {inside.mean():.3f} of the ring buffer; kept {np.mean(b[..., 3] < 1):.4f}")
+    np.testing.assert_array_equal(a[inside], b[inside])
+    # outside the box the whole-buffer pass culled every ray
+    assert np.all(b[~inside][:, 3] == 1.0) and np.all(b[~inside][:, :3] == 0.0)
+    assert np.any(b[inside][:, 3] < 1.0)
+    # the composite blends the ring only where the fan periapsis is low
+    assert np.all(np.isfinite(with_ring))
+    assert np.mean(np.all(with_ring == no_ring, axis=-1)) > 0.5
+
+
+def test_row_bands_assemble_the_full_frame(hip):
+    W, H = 96, 64
+    print("case: banded layered 96x64", flush=True)
+    disc = scenes.config_disc()
+    ctx = context(W, H, disc=disc)
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=1025, n_fine=257)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    layer = layered.Image(ctx, W, H)
+    layered.render_layered(ctx, sky, cam, SCH, layer)
+    full = ctx.download()
+    ctx.clear()
+    layered.render_layered(ctx, sky, cam, SCH, layer, row0=0, rows=40)
+    layered.render_layered(ctx, sky, cam, SCH, layer, row0=40, rows=24)
+    banded = ctx.download()
+    # accumulate adds a second pass on top
+    layered.render_layered(ctx, sky, cam, SCH, layer, accumulate=True)
+    twice = ctx.download()
+    with pytest.raises(ValueError):
+        small = layered.Image(ctx, W // 2, H // 2)
+        layered.render_layered(ctx, sky, cam, SCH, small, row0=0, rows=32)
+    for o in (layer, sky, small):
+        o.close()
+    ctx.close()
+    np.testing.assert_array_equal(full, banded)
+    np.testing.assert_allclose(twice, 2 * full, rtol=1e-6, atol=1e-7)
+
+
+def test_layered_argument_errors(hip):
+    ctx = context(32, 32)
+    cam = camera()
+    sky = layered.SkyFanState(ctx, n=64, n_fine=16)
+    layered.update_sky_fan(sky, ctx, cam.pos, SCH, gate=22.0)
+    rgb = layered.Image(ctx, 32, 32, channels=3)
+    lay = layered.Image(ctx, 32, 32)
+    c32, c64 = api.ks_cam_params(cam, M)
+    sp = api.spacetime_params(SCH, cam, api.StepRule(dt=DT))
+    rule = api.StepRule(dt=DT)
+    cases = [
+        (rgb, api.trace_args(32, 32, rule), 1, L.ST_ERR_INVALID_ARG),           # not a layer
+        (lay, api.trace_args(16, 32, rule), 1, L.ST_ERR_INVALID_ARG),           # size mismatch
+        (lay, api.trace_args(32, 32, rule, precision=L.ST_PREC_FP64), 1, L.ST_ERR_UNSUPPORTED),
+        (lay, api.trace_args(32, 32, rule), 0, L.ST_ERR_INVALID_ARG),           # substride 0
+    ]
+    for img, args, sub, code in cases:
+        with pytest.raises(L.StError) as e:
+            L.check(L.lib().st_trace_layer(ctx.handle, sky.handle, img.handle, L.fptr(c32), L.dptr(c64),
+                                           L.fptr(sp), L.C.byref(args), 0.0, sub, 0, 0))
+        assert e.value.code == code
+    with pytest.raises(L.StError) as e:
+        L.check(L.lib().st_sky_composite(ctx.handle, sky.handle, rgb.handle, None, L.fptr(c32), L.fptr(sp), 0.0,
+                                         0.5, 0.5, 0, 0, -1))
+    assert e.value.code == L.ST_ERR_INVALID_ARG
+    with pytest.raises(L.StError):
+        L.check(L.lib().st_skyfan_create(ctx.handle, 2, 16, L.C.byref(L.C.c_void_p())))
+    for o in (rgb, lay, sky):
+        o.close()
+    ctx.close()
