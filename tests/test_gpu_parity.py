@@ -370,8 +370,8 @@ def test_bad_arguments_are_error_codes(hip):
         ctx.trace(c32, c64, sp, args)
     assert e.value.code == L.ST_ERR_INVALID_ARG
     args = api.trace_args(32, 16, sc.rule)
-    args.stat = 1                        # refinement statistics variant not built
+    args.stat = 1                        # refinement moments: st_trace only, not the depth-bucket trace
     with pytest.raises(L.StError) as e:
-        ctx.trace(c32, c64, sp, args)
+        L.check(L.lib().st_trace_depth(ctx.handle, L.fptr(c32), L.dptr(c64), L.fptr(sp), L.C.byref(args), 4))
     assert e.value.code == L.ST_ERR_UNSUPPORTED
     ctx.close()
