@@ -178,18 +178,21 @@ __global__ __launch_bounds__(256) void k_trace_layer(const DeviceScene* __restri
     tile_pixel(G, blockIdx.x, threadIdx.x, di, dj, valid);
     const int i = di * substride + subx, j = dj * substride + suby;
     valid = valid && i < lw && j < lh;
-    unsigned long long steps = 0;
+    unsigned long long steps = 0, gas = 0;
     if (valid) {
         RayOut<float> o;
         trace_ray<float, KERR, VOL, false, false, false, true>(D, P, F, i, j, o);
         reinterpret_cast<float4*>(F.out)[(size_t)j * lw + i] = make_float4(o.rgb[0], o.rgb[1], o.rgb[2], o.alpha);
         steps = (unsigned long long)o.steps;
+        gas = (unsigned long long)o.gas_samples;
     }
     const unsigned long long tot = wave_sum(steps);
     const unsigned long long nv = wave_sum(valid ? 1ull : 0ull);
+    const unsigned long long ng = VOL ? wave_sum(gas) : 0ull;
     if ((threadIdx.x & 63) == 0) {
         if (tot) atomicAdd(&P.counters[0], tot);
         if (nv) atomicAdd(&P.counters[3], nv);
+        if (ng) atomicAdd(&P.counters[4], ng);
     }
 }
 
