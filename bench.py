@@ -596,7 +596,7 @@ def cpu_baseline(sc):
     o.render("rk4", row0=0, rows=rows, nthreads=cores)
     t_rk4 = time.perf_counter() - t0
     rk4_mrays = sc.width * rows / t_rk4 / 1e6
-    trows = max(8, sc.height // 16)
+    trows = max(8, sc.height // 4)
     r0 = sc.height // 2 - trows // 2
     t0 = time.perf_counter()
     o.render("tsit5", tol=1e-6, row0=r0, rows=trows, nthreads=cores)
