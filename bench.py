@@ -90,6 +90,15 @@ def header(args, W, H, rays_per_step, wall, workload, dtype):
     }
 
 
+def launch_shape(args, bt, targs):
+    """The batch launch shape: autotuned, or fixed by --shape TILE,WAVES
+    (WAVES -1 = one wave per tile) to skip the sweep, e.g. under a profiler."""
+    if args.shape:
+        t, w = (int(x) for x in args.shape.split(","))
+        return t, w, 0.0
+    return bt.autotune(targs)
+
+
 # ---------------------------------------------------------------------------
 # Kerr a=0.9 1080p frame (the headline)
 # ---------------------------------------------------------------------------
@@ -245,11 +254,11 @@ def bench_batch(args):
     sp = api.spacetime_params(b.spacetime, b.cameras[0], b.rule)
     bt = api.BatchTrace(ctx, c32, c64, b.sizes, sp)
     targs = api.trace_args(8, 8, b.rule, precision=prec)
-    t, w, tune_ms = bt.autotune(targs)
+    t, w, tune_ms = launch_shape(args, bt, targs)
     rays = sum(x * y for x, y in b.sizes)
 
     def step():
-        bt.trace(targs, graph=True)
+        bt.trace(targs, tile=t, waves_per_cu=w, graph=True)
 
     wall = timed(step, ctx.sync, args.steps, args.warmup)
     kern, st = [], None
@@ -263,7 +272,7 @@ def bench_batch(args):
                   {"fp32": "f32", "mixed": "f32+f64"}[args.precision])
     line["metric"] = "Mrays/s, 64-camera batched free-fall trace"
     line["config"]["autotuned"] = {"tile": t, "waves_per_cu": w, "ms": round(tune_ms, 3),
-                                   "one_shot": w == L.ST_BATCH_ONE_SHOT}
+                                   "one_shot": w == L.ST_BATCH_ONE_SHOT, "fixed_by_flag": args.shape is not None}
     line["roofline"] = roofline(st["steps_fp32"] + st["steps_fp64"], st["rays"], float(np.mean(kern)), F_STEP_SCHW,
                                 F_RAY_SCHW)
     line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
@@ -291,11 +300,11 @@ def bench_volume(args):
     sizes = [(sc.width, sc.height)] * len(cams)
     bt = api.BatchTrace(ctx, c32, c64, sizes, sp)
     targs = api.trace_args(8, 8, sc.rule, precision=prec, gas_ds=scenes.GAS_DS_256)
-    t, w, tune_ms = bt.autotune(targs)
+    t, w, tune_ms = launch_shape(args, bt, targs)
     rays = sum(x * y for x, y in sizes)
 
     def step():
-        bt.trace(targs, graph=True)
+        bt.trace(targs, tile=t, waves_per_cu=w, graph=True)
 
     wall = timed(step, ctx.sync, args.steps, args.warmup)
     kern, st = [], None
@@ -309,7 +318,7 @@ def bench_volume(args):
                   {"fp32": "f32", "mixed": "f32+f64"}[args.precision])
     line["metric"] = "Mrays/s, 4K volumetric Kerr a=0.99, 8 frames per launch"
     line["config"]["autotuned"] = {"tile": t, "waves_per_cu": w, "ms": round(tune_ms, 3),
-                                   "one_shot": w == L.ST_BATCH_ONE_SHOT}
+                                   "one_shot": w == L.ST_BATCH_ONE_SHOT, "fixed_by_flag": args.shape is not None}
     kms = float(np.mean(kern))
     # the march reads 8 grid taps x 4 B per density sample (algorithmic)
     gas_bytes = st["gas_samples"] * GAS_BYTES_PER_SAMPLE
@@ -320,6 +329,9 @@ def bench_volume(args):
                         "gas_samples_per_ray": round(st["gas_samples"] / max(st["rays"], 1), 2),
                         "note": "algorithmic gathers: 32 B per trilinear density sample; the 32 MiB grid is "
                                 "mostly served from L2/MALL, so this can exceed the HBM rate"}
+    tfile = ROOT / "profiles" / "traffic_volume4k_kerr099.json"
+    if tfile.exists() and args.precision == "fp32":
+        line["roofline"]["traffic"] = json.loads(tfile.read_text()).get("bytes_per_launch")
     line["roofline_valu"] = roofline(st["steps_fp32"] + st["steps_fp64"], st["rays"], kms, F_STEP_KERR, F_RAY_KERR)
     line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
     line["cpu_baseline"] = None
@@ -631,6 +643,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the mixed / surrogate / D2H legs")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--shape", default=None, help="batch workloads: fixed launch shape TILE,WAVES (no autotune)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
