@@ -83,7 +83,7 @@ _SIG = {
     "ora_tsit5_tableau": (None, [_D, _D, _D, _D]),
     "ora_sky_fan": (None, [_D, _D, C.c_int, C.c_double, C.c_int, C.c_int, _D, _D]),
     "ora_fan_band": (None, [_D, C.c_int, C.c_int, _D]),
-    "ora_trace_layer": (None, [C.POINTER(OraScene), C.c_int, C.c_int, _D, C.c_int, _D, C.c_double, _D]),
+    "ora_trace_layer": (None, [C.POINTER(OraScene), C.c_int, C.c_int, _D, C.c_int, _D, C.c_double, _D, _I]),
     "ora_sky_composite": (None, [C.POINTER(OraScene), _D, C.c_int, _D, C.c_int, _D, _F, C.c_int, C.c_int, _F,
                                  C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _D]),
 }
@@ -300,16 +300,18 @@ class Scene:
                               out.ctypes.data_as(_D))
         return out
 
-    def trace_layer(self, i: int, j: int, fan: np.ndarray, skyp, ring_rmin: float = 0.0) -> np.ndarray:
+    def trace_layer(self, i: int, j: int, fan: np.ndarray, skyp, ring_rmin: float = 0.0,
+                    with_steps: bool = False):
         """The LAYER output (premultiplied emission, transmittance) of pixel
         (i, j) against a fan (n, 4) and sky block; this scene's width/height
-        are the sensor plane."""
+        are the sensor plane.  with_steps: also the integrator steps."""
         f = np.ascontiguousarray(fan, dtype=np.float64)
         sp = np.ascontiguousarray(skyp, dtype=np.float64)
         out = np.zeros(4)
+        steps = C.c_int(0)
         lib().ora_trace_layer(self.ptr, int(i), int(j), f.ctypes.data_as(_D), f.shape[0], sp.ctypes.data_as(_D),
-                              float(ring_rmin), out.ctypes.data_as(_D))
-        return out
+                              float(ring_rmin), out.ctypes.data_as(_D), C.byref(steps))
+        return (out, steps.value) if with_steps else out
 
     def sky_composite(self, fan, fine, skyp, layer: np.ndarray, ring: np.ndarray | None = None,
                       ring_rmin: float = 0.0, row0: int = 0, rows: int | None = None) -> np.ndarray:

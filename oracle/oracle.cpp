@@ -1506,11 +1506,13 @@ void ora_fan_band(const double* fan, int n, int pad, double skyp[12]) {
  * 1613-1627): out[4] = premultiplied emission, transmittance.  The scene's
  * width/height are the sensor plane. */
 void ora_trace_layer(const ora_scene* sc, int i, int j, const double* fan, int n, const double skyp[12],
-                     double ring_rmin, double out[4]) {
+                     double ring_rmin, double out[4], int* steps) {
     LayerIn lay{fan, n, skyp, ring_rmin, {0, 0, 0, 1}};
     ora_ray o;
+    std::memset(&o, 0, sizeof(o));
     trace_rk4(sc, i, j, &o, 0, nullptr, &lay);
     for (int c = 0; c < 4; ++c) out[c] = lay.out[c];
+    if (steps) *steps = o.steps;
 }
 
 }  // extern "C"

@@ -171,9 +171,10 @@ void ora_sky_fan(const double cam[20], const double skyp[12], int nmax, double d
 /* fan_band_kernel! (src/metal.jl:2888-2898): the band into skyp[8..9]. */
 void ora_fan_band(const double* fan, int n, int pad, double skyp[12]);
 /* The tracer's LAYER output for pixel (i, j): out[4] = premultiplied emission,
- * transmittance (src/metal.jl:1095-1135, 1202-1211, 1613-1627). */
+ * transmittance (src/metal.jl:1095-1135, 1202-1211, 1613-1627); steps (may be
+ * null) receives the integrator steps taken. */
 void ora_trace_layer(const ora_scene* sc, int i, int j, const double* fan, int n, const double skyp[12],
-                     double ring_rmin, double out[4]);
+                     double ring_rmin, double out[4], int* steps);
 /* sky_composite_kernel! (src/metal.jl:2707-2878) over rows [row0, row0+rows)
  * of the scene's frame: out[H][W][3] assigned.  layer/ring float[h][w][4]. */
 void ora_sky_composite(const ora_scene* sc, const double* fan, int n, const double* fine, int nf,

@@ -212,17 +212,21 @@ def test_layer_matches_oracle(hip, case):
     ctx.close()
     o = oracle_frame_scene(ctx, cam, W, H, disc=disc, a=a, volume=vol)
     fan64, skp64 = fan.astype(np.float64), skp.astype(np.float64)
-    want = np.array([[o.trace_layer(i, j, fan64, skp64) for i in range(W)] for j in range(H)])
-    culled_g = (got[..., 3] == 1.0) & np.all(got[..., :3] == 0.0, axis=-1)
-    culled_o = (want[..., 3] == 1.0) & np.all(want[..., :3] == 0.0, axis=-1)
-    agree = np.mean(culled_g == culled_o)
-    live = ~culled_o
+    res = [[o.trace_layer(i, j, fan64, skp64, with_steps=True) for i in range(W)] for j in range(H)]
+    want = np.array([[r[0] for r in row] for row in res])
+    osteps = np.array([[r[1] for r in row] for row in res])
+    culled = osteps == 0               # the oracle's fan cull: no step taken
+    clear_g = (got[..., 3] == 1.0) & np.all(got[..., :3] == 0.0, axis=-1)
+    clear_o = (want[..., 3] == 1.0) & np.all(want[..., :3] == 0.0, axis=-1)
+    agree = np.mean(clear_g == clear_o)
+    live = ~clear_o
     rel = (np.abs(got - want) / np.maximum(np.abs(want), 1e-2)).max(-1)[live]
-    print(f"culled {culled_o.mean():.3f} agreement {agree:.5f}; live rel p50 {np.median(rel):.2e} "
-          f"p97 {np.percentile(rel, 97):.2e} max {rel.max():.2e}; rays {stats['rays']} steps {stats['steps_fp32']} "
-          f"gas samples {stats['gas_samples']}")
+    print(f"culled {culled.mean():.3f}, clear {clear_o.mean():.3f} (agreement {agree:.5f}); live rel p50 "
+          f"{np.median(rel):.2e} p97 {np.percentile(rel, 97):.2e} max {rel.max():.2e}; rays {stats['rays']} steps "
+          f"{stats['steps_fp32']} (oracle {osteps.sum()}) gas samples {stats['gas_samples']}")
     assert stats["rays"] == W * H
-    assert 0.05 < culled_o.mean() < 0.95 and agree > 0.998
+    assert 0.05 < culled.mean() < 0.95 and clear_g[culled].all() and agree > 0.998
+    assert abs(stats["steps_fp32"] - osteps.sum()) < 0.01 * osteps.sum()
     assert (got[..., :3] > 0).any()
     if vol is None:
         assert np.median(rel) < 1e-5 and np.percentile(rel, 99) < 1e-3
