@@ -457,7 +457,8 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
                      focus_dist: float = 1.0, relativistic: bool = False, pixel_jitter: bool = True,
                      order: int = 4, tol: float = 1e-4, rule: Optional[StepRule] = None,
                      precision: int = L.ST_PREC_FP32, refine: int = 1, refine_var: float = 0.35,
-                     refine_graze: float = 0.05, refine_margin: int = 8, return_box: bool = False):
+                     refine_graze: float = 0.05, refine_margin: int = 8, return_box: bool = False,
+                     fixed_strata: bool = False):
     """render_draft_mtl (src/metal.jl:2040-2242): samples^2 stratified passes,
     per-pixel decorrelated jitter, optional thin-lens strata, and adaptive
     refinement: with refine > 1 the base passes also accumulate per-pixel
@@ -467,7 +468,9 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
     refine_margin pixels) is re-rendered refine - 1 more times with fresh
     strata, its pixels then scaled by 1/refine.  Motion blur is not built.
     With return_box the refined box (col0, ncols, row0, nrows) or None is
-    returned as well."""
+    returned as well.  fixed_strata (no pixel jitter): every pass at its
+    stratum's centre, as the reference CPU render's jittered = false
+    (src/raytrace.jl:380-395)."""
     _check_size(ctx, width, height)
     rng = rng if rng is not None else np.random.default_rng()
     if rule is None:
@@ -480,7 +483,10 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
     weight = 1.0 / samples ** 2
     jw = 1.0 / samples if pixel_jitter else 0.0
     cells = [(a, b) for b in range(samples) for a in range(samples)]
-    if pixel_jitter:
+    if fixed_strata:
+        jw = 0.0
+        offs = [((a + 0.5) / samples, (b + 0.5) / samples) for a, b in cells]
+    elif pixel_jitter:
         # stratum origins; each pixel hashes its own point inside the stratum
         offs = [(cells[k][0] / samples, cells[k][1] / samples) for k in rng.permutation(len(cells))]
     elif samples == 1:
@@ -577,19 +583,45 @@ def render_depth_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
     return ctx.depth(nbuckets)
 
 
-def render(cam: Camera, spacetime, background: np.ndarray, *, disc: Optional[AccretionDisc] = None,
-           volume: Optional[DiscVolume] = None, width: int = 320, height: int = 180, samples: int = 1,
-           relativistic: bool = False, rule: Optional[StepRule] = None, precision: int = L.ST_PREC_MIXED,
-           rng: Optional[np.random.Generator] = None) -> np.ndarray:
-    """render(cam, bh, sky; disc, width, height) (src/raytrace.jl:339-428) on
-    the GPU: FP32 bulk with FP64 promotion by default."""
+_DEFAULT_DISC = object()
+
+
+def render(cam: Camera, spacetime, background: np.ndarray, *, disc=_DEFAULT_DISC, dust=None,
+           volume: Optional[DiscVolume] = None, width: int = 400, height: int = 200, solver: str = "rk4",
+           samples: int = 1, jittered: bool = True, rng: Optional[np.random.Generator] = None, progress=None,
+           relativistic: bool = False, starfield: Optional[dict] = None, rule: Optional[StepRule] = None,
+           precision: int = L.ST_PREC_MIXED) -> np.ndarray:
+    """render(cam, bh, sky; disc, dust, volume, width, height, solver,
+    samples, jittered, rng, progress, relativistic, starfield)
+    (src/raytrace.jl:339-428) on the GPU, FP32 bulk with FP64 promotion by
+    default; returns (height, width, 3) float32.
+
+    Keywords as the reference's: `disc` defaults to AccretionDisc() (None:
+    no disc); samples^2 strata per pixel, jittered per pixel or at the
+    stratum centres; `progress(fraction)` after each pass; `starfield` is the
+    keyword set of HipPreviewContext.set_starfield (None: off).  `solver`
+    names the integrator: "rk4" (the GPU kernel's fixed-step rule) or "tsit5"
+    (its adaptive Tsitouras 5(4) at the reference's reltol 1e-6); the
+    reference's DifferentialEquations.jl solvers (Vern9) are not part of this
+    library.  Interstellar dust is not part of this library either: `dust`
+    must be None."""
+    if dust is not None:
+        raise NotImplementedError("interstellar dust (src/dust.jl) is not part of this library")
+    if solver not in ("rk4", "tsit5"):
+        raise ValueError(f"solver must be 'rk4' or 'tsit5', not {solver!r}")
+    if disc is _DEFAULT_DISC:
+        disc = AccretionDisc()
     ctx = HipPreviewContext(background, width, height, disc=disc, volume=volume)
     try:
+        if starfield is not None:
+            ctx.set_starfield(**starfield)
         if rule is None:
             rule = StepRule.reference()
+        order, tol = (4, rule.tol) if solver == "rk4" else (45, 1e-6)
         return render_draft_hip(ctx, cam, spacetime, width=width, height=height, samples=samples,
                                 relativistic=relativistic, rule=rule, precision=precision, rng=rng,
-                                pixel_jitter=samples > 1)
+                                progress=progress, pixel_jitter=jittered,
+                                fixed_strata=not jittered, order=order, tol=tol)
     finally:
         ctx.close()
 

@@ -1,0 +1,58 @@
+"""The reference CPU `render` signature (src/raytrace.jl:339-428) on the GPU
+path: strata at their centres when not jittered, the progress callback, the
+starfield keywords, the two integrators, and the rejected dust keyword."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from spacetime_amd import _lib as L
+from spacetime_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_unjittered_render_is_the_mean_of_the_stratum_centres(hip):
+    sc = scenes.kerr09_1080p(96, 54)
+    print("case: render 96x54 samples=2 jittered=False", flush=True)
+    img = api.render(sc.camera, sc.spacetime, scenes.sky(), disc=sc.disc, width=96, height=54, samples=2,
+                     jittered=False, precision=L.ST_PREC_FP32)
+    ctx = api.HipPreviewContext(scenes.sky(), 96, 54, disc=sc.disc)
+    rule = api.StepRule.reference()
+    c32, c64 = api.ks_cam_params(sc.camera, 1.0, 0.9)
+    sp = api.spacetime_params(sc.spacetime, sc.camera, rule)
+    ctx.clear()
+    for b in range(2):
+        for a in range(2):
+            ctx.trace(c32, c64, sp, api.trace_args(96, 54, rule, ju=(a + 0.5) / 2, jv=(b + 0.5) / 2, weight=0.25))
+    want = ctx.download()
+    ctx.close()
+    np.testing.assert_array_equal(img, want)
+    assert img.shape == (54, 96, 3) and img.max() > 0
+
+
+def test_render_keywords(hip):
+    sc = scenes.kerr09_1080p(64, 36)
+    print("case: render keywords 64x36", flush=True)
+    seen = []
+    kw = dict(width=64, height=36, precision=L.ST_PREC_FP32)
+    a = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=2, progress=seen.append,
+                   rng=np.random.default_rng(1), **kw)
+    assert seen and seen[-1] == pytest.approx(1.0) and all(np.diff(seen) >= 0)
+    b = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=2, rng=np.random.default_rng(1), **kw)
+    np.testing.assert_array_equal(a, b)          # same rng, same image
+    c = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=2, rng=np.random.default_rng(2), **kw)
+    assert not np.array_equal(a, c)               # jittered: another rng, other strata points
+    s = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=2, rng=np.random.default_rng(1),
+                   starfield=dict(strength=1.0, texture_weight=0.0, height=36), **kw)
+    assert not np.array_equal(a, s) and np.all(np.isfinite(s))
+    nodisc = api.render(sc.camera, sc.spacetime, scenes.sky(), disc=None, samples=1, jittered=False, **kw)
+    plain = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=1, jittered=False, **kw)
+    assert (plain.sum(-1) > nodisc.sum(-1) + 1e-3).any()   # the default disc is on
+    ts = api.render(sc.camera, sc.spacetime, scenes.sky(), samples=1, jittered=False, solver="tsit5", **kw)
+    dark_t, dark_r = ts.sum(-1) < 1e-6, plain.sum(-1) < 1e-6
+    assert np.mean(dark_t == dark_r) > 0.98 and np.all(np.isfinite(ts))
+    with pytest.raises(NotImplementedError):
+        api.render(sc.camera, sc.spacetime, scenes.sky(), dust=object(), **kw)
+    with pytest.raises(ValueError):
+        api.render(sc.camera, sc.spacetime, scenes.sky(), solver="vern9", **kw)
