@@ -458,15 +458,19 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
                      order: int = 4, tol: float = 1e-4, rule: Optional[StepRule] = None,
                      precision: int = L.ST_PREC_FP32, refine: int = 1, refine_var: float = 0.35,
                      refine_graze: float = 0.05, refine_margin: int = 8, return_box: bool = False,
-                     fixed_strata: bool = False):
+                     fixed_strata: bool = False, camera_at=None, per_pixel_shutter: bool = False):
     """render_draft_mtl (src/metal.jl:2040-2242): samples^2 stratified passes,
     per-pixel decorrelated jitter, optional thin-lens strata, and adaptive
-    refinement: with refine > 1 the base passes also accumulate per-pixel
+    refinement, and motion blur: camera_at(s) (shutter fraction s in [0, 1))
+    gives each pass the camera of its own shutter stratum (permuted), at the
+    stratum's midpoint, or with per_pixel_shutter both stratum ends so each
+    pixel hashes its own instant between them; `cam` stays the nominal pose
+    that sets the escape radius.  With refine > 1 the base passes also accumulate per-pixel
     luminance moments and the photon-shell graze fraction; pixels whose
     per-ray sigma exceeds refine_var * (mean + 0.01) or whose graze fraction
     exceeds refine_graze are flagged, and the flagged bounding box (plus
     refine_margin pixels) is re-rendered refine - 1 more times with fresh
-    strata, its pixels then scaled by 1/refine.  Motion blur is not built.
+    strata, its pixels then scaled by 1/refine.
     With return_box the refined box (col0, ncols, row0, nrows) or None is
     returned as well.  fixed_strata (no pixel jitter): every pass at its
     stratum's centre, as the reference CPU render's jittered = false
@@ -495,6 +499,12 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
         # jittered_grid: one random point per stratum, shared by the frame
         offs = [((a + rng.random()) / samples, (b + rng.random()) / samples) for a, b in cells]
     lens_perm = rng.permutation(samples ** 2)
+    time_perm = rng.permutation(samples ** 2)
+    a_spin = spin(spacetime)
+
+    def pose(s_frac):
+        return ks_cam_params(camera_at(s_frac), M, a_spin, fisheye_deg=fisheye_deg, focus_dist=focus_dist)
+
     ndispatch = len(offs) * max(refine, 1)
     done = 0
     pass_base = 0
@@ -505,7 +515,19 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
         nonlocal done, pass_base
         for p, (du, dv) in enumerate(offs):
             pss = pass_base + p + 1
-            cp32, cp64 = c32.copy(), c64.copy()
+            if camera_at is not None:
+                # this pass's shutter stratum (src/metal.jl:2136-2158)
+                m, n = int(time_perm[p]), samples ** 2
+                if per_pixel_shutter:
+                    cp32, cp64 = pose(m / n)
+                    e32, e64 = pose((m + 1) / n)
+                    for arr, e in ((cp32, e32), (cp64, e64)):
+                        arr[28] = 104729 + pss      # seed clear of the lens hashes
+                        arr[29:49] = e[0:20]
+                else:
+                    cp32, cp64 = pose((m + 0.5) / n)
+            else:
+                cp32, cp64 = c32.copy(), c64.copy()
             if use_dof:
                 m = int(lens_perm[p])
                 for arr in (cp32, cp64):

@@ -56,3 +56,31 @@ def test_render_keywords(hip):
         api.render(sc.camera, sc.spacetime, scenes.sky(), dust=object(), **kw)
     with pytest.raises(ValueError):
         api.render(sc.camera, sc.spacetime, scenes.sky(), solver="vern9", **kw)
+
+
+def test_motion_blur_strata(hip):
+    """camera_at: one stratum renders from its midpoint pose; a camera that
+    does not move gives the plain image, with or without the per-pixel
+    shutter (the pose between equal ends is exact)."""
+    sc = scenes.kerr09_1080p(80, 45)
+    print("case: motion blur 80x45", flush=True)
+    ctx = api.HipPreviewContext(scenes.sky(), 80, 45, disc=sc.disc)
+    p0, p1 = np.array([0.0, -30.0, 3.0]), np.array([2.0, -29.0, 3.5])
+    moving = lambda s: api.Camera(p0 + s * (p1 - p0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0))
+    kw = dict(width=80, height=45, samples=1, pixel_jitter=False)
+    # the nominal camera (the positional one) sets the escape radius
+    blur = api.render_draft_hip(ctx, moving(0.5), sc.spacetime, camera_at=moving, rng=np.random.default_rng(3),
+                                **kw)
+    mid = api.render_draft_hip(ctx, moving(0.5), sc.spacetime, rng=np.random.default_rng(3), **kw)
+    np.testing.assert_array_equal(blur, mid)
+    still = lambda s: sc.camera
+    kw2 = dict(width=80, height=45, samples=2)
+    plain = api.render_draft_hip(ctx, sc.camera, sc.spacetime, rng=np.random.default_rng(4), **kw2)
+    for ppx in (False, True):
+        b = api.render_draft_hip(ctx, sc.camera, sc.spacetime, camera_at=still, per_pixel_shutter=ppx,
+                                 rng=np.random.default_rng(4), **kw2)
+        np.testing.assert_array_equal(b, plain)
+    pp = api.render_draft_hip(ctx, sc.camera, sc.spacetime, camera_at=moving, per_pixel_shutter=True,
+                              rng=np.random.default_rng(4), **kw2)
+    ctx.close()
+    assert np.all(np.isfinite(pp)) and not np.array_equal(pp, plain)
