@@ -93,6 +93,37 @@ class Camera:
         self.velocity = np.asarray(velocity, dtype=np.float64)
 
 
+class ThinLensCamera:
+    """ThinLensCamera(pos, target, up; focal_length, sensor_width, f_number,
+    focus_distance, velocity) (src/camera.jl:66-92): focal length and sensor
+    width in mm, aperture = focal_length / f_number (mm), focus distance in
+    world units."""
+
+    def __init__(self, pos, target, up, focal_length=50.0, sensor_width=36.0, f_number=2.8, focus_distance=100.0,
+                 velocity=(0.0, 0.0, 0.0)):
+        self.pos = np.asarray(pos, dtype=np.float64)
+        self.target = np.asarray(target, dtype=np.float64)
+        self.up = np.asarray(up, dtype=np.float64)
+        self.fwd = _normalize(self.target - self.pos)
+        self.right = _normalize(np.cross(self.fwd, self.up))
+        self.up_local = np.cross(self.right, self.fwd)
+        self.focal_length = float(focal_length)
+        self.sensor_width = float(sensor_width)
+        self.fov_factor = (self.sensor_width / 2.0) / self.focal_length
+        self.aperture = self.focal_length / float(f_number)
+        self.focus_distance = float(focus_distance)
+        self.velocity = np.asarray(velocity, dtype=np.float64)
+
+    def pinhole(self) -> Camera:
+        """The pinhole the GPU paths trace (src/metal.jl:2244-2252): same pose
+        and field of view (the reference drops the velocity here)."""
+        return Camera(self.pos, self.pos + self.fwd, self.up_local, self.fov_factor)
+
+    def aperture_world(self) -> float:
+        """World-space aperture diameter, focus / f_number (src/metal.jl:2248-2249)."""
+        return self.aperture * self.focus_distance / self.focal_length
+
+
 def camera_beta(cam: Camera) -> np.ndarray:
     return np.array([cam.velocity @ cam.fwd, cam.velocity @ cam.right, cam.velocity @ cam.up_local])
 
@@ -432,7 +463,10 @@ def render_preview_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, fishey
                        relativistic: bool = False, order: int = 4, tol: float = 1e-4,
                        rule: Optional[StepRule] = None, precision: int = L.ST_PREC_FP32) -> np.ndarray:
     """render_preview_mtl (src/metal.jl:1790-1881): one full-frame dispatch of
-    centre-of-pixel rays; returns (H, W, 3) float32."""
+    centre-of-pixel rays; returns (H, W, 3) float32.  A ThinLensCamera is
+    traced as its pinhole (src/metal.jl:2254-2259)."""
+    if isinstance(cam, ThinLensCamera):
+        cam = cam.pinhole()
     if rule is None:
         rule = StepRule(dt=ctx.dt, r_escape_factor=ctx.r_escape_factor, order=order, tol=tol)
         M = spacetime.M
@@ -475,6 +509,14 @@ def render_draft_hip(ctx: HipPreviewContext, cam: Camera, spacetime, *, width: i
     returned as well.  fixed_strata (no pixel jitter): every pass at its
     stratum's centre, as the reference CPU render's jittered = false
     (src/raytrace.jl:380-395)."""
+    if isinstance(cam, ThinLensCamera):
+        return render_draft_hip(ctx, cam.pinhole(), spacetime, width=width, height=height, samples=samples, dt=dt,
+                                rng=rng, progress=progress, fisheye_deg=fisheye_deg,
+                                aperture_world=cam.aperture_world(), focus_dist=cam.focus_distance,
+                                relativistic=relativistic, pixel_jitter=pixel_jitter, order=order, tol=tol,
+                                rule=rule, precision=precision, refine=refine, refine_var=refine_var,
+                                refine_graze=refine_graze, refine_margin=refine_margin, return_box=return_box,
+                                fixed_strata=fixed_strata, camera_at=camera_at, per_pixel_shutter=per_pixel_shutter)
     _check_size(ctx, width, height)
     rng = rng if rng is not None else np.random.default_rng()
     if rule is None:

@@ -84,3 +84,23 @@ def test_motion_blur_strata(hip):
                               rng=np.random.default_rng(4), **kw2)
     ctx.close()
     assert np.all(np.isfinite(pp)) and not np.array_equal(pp, plain)
+
+
+def test_thin_lens_camera_is_its_pinhole_with_the_aperture(hip):
+    sc = scenes.kerr09_1080p(64, 36)
+    print("case: thin lens 64x36", flush=True)
+    tl = api.ThinLensCamera((0.0, -30.0, 3.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), focal_length=24.0,
+                            f_number=1.4, focus_distance=30.0)
+    ctx = api.HipPreviewContext(scenes.sky(), 64, 36, disc=sc.disc)
+    kw = dict(width=64, height=36, samples=2)
+    a = api.render_draft_hip(ctx, tl, sc.spacetime, rng=np.random.default_rng(9), **kw)
+    b = api.render_draft_hip(ctx, tl.pinhole(), sc.spacetime, aperture_world=24.0 / 1.4 * 30.0 / 24.0,
+                             focus_dist=30.0, rng=np.random.default_rng(9), **kw)
+    c = api.render_draft_hip(ctx, tl.pinhole(), sc.spacetime, rng=np.random.default_rng(9), **kw)
+    p = api.render_preview_hip(ctx, tl, sc.spacetime)
+    q = api.render_preview_hip(ctx, tl.pinhole(), sc.spacetime)
+    ctx.close()
+    assert tl.fov_factor == pytest.approx(0.75)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)          # the aperture blurs
+    np.testing.assert_array_equal(p, q)
