@@ -18,6 +18,7 @@ device-to-host copy of each frame.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -135,6 +136,19 @@ def bench_frame(args):
     tfile = ROOT / "profiles" / "traffic_kerr09_1080p_fp32.json"
     if tfile.exists() and prec == L.ST_PREC_FP32 and (W, H) == (1920, 1080):
         line["roofline"]["traffic"] = json.loads(tfile.read_text()).get("bytes_per_launch")
+    # the achievable non-packed FMA rate, measured on this device now, and the
+    # FP32 arithmetic the kernel executes (hardware FLOPs, FMA = 2, from the
+    # committed PMC counts of this launch) against it
+    peak = ctypes.c_float()
+    L.check(L.lib().st_measure_fp32_fma_peak(ctx.handle, ctypes.byref(peak)))
+    hw = {"fp32_fma_peak_measured_tflops": round(peak.value, 2)}
+    vfile = ROOT / "profiles" / "valu_kerr09_1080p_fp32.json"
+    if vfile.exists() and prec == L.ST_PREC_FP32 and (W, H) == (1920, 1080):
+        v = json.loads(vfile.read_text())
+        hw_tf = v["flops_fp32_per_lane_per_dispatch"] * 64 * v["valu_utilization"] / (kernel_ms * 1e-3) / 1e12
+        hw["fp32_tflops_executed"] = round(hw_tf, 2)
+        hw["frac_of_measured_fma_peak"] = round(hw_tf / peak.value, 3)
+    line["roofline"]["hardware"] = hw
     line["promoted_fraction"] = round(st["promoted"] / max(st["rays"], 1), 5)
 
     if not args.no_legs:

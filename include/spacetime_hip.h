@@ -288,6 +288,12 @@ int st_sky_composite(st_ctx* ctx, const st_skyfan* fan, const st_image* layer, c
                      const float cam_params[49], const float spacetime_params[8], float ring_rmin, float ju,
                      float jv, int accumulate, int row0, int rows);
 
+/* Measured FP32 vector-FMA rate of the context's device in TFLOP/s: 16
+ * independent fmaf chains per lane over 2M lanes, non-packed (the form
+ * every kernel of this library is built for).  Synchronises.  No counterpart
+ * in the reference: the benchmark reports it beside the datasheet peak. */
+int st_measure_fp32_fma_peak(st_ctx* ctx, float* tflops);
+
 /* ---- live fluid-disc simulation (src/disc_sim.jl) ----
  * A 2D stable-fluids solver on the gas volume's (log s, phi) grid (Stam
  * 1999): semi-Lagrangian advection of a perturbation velocity under the

@@ -131,6 +131,7 @@ SIGNATURES = {
                                    C.c_float, C.c_int, C.c_int, C.c_int]),
     "st_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_char_p, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "st_measure_fp32_fma_peak": (C.c_int, [C.c_void_p, _F]),
     "st_mlp_param_count": (C.c_size_t, []),
     "st_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
     "st_mlp_destroy": (C.c_int, [C.c_void_p]),
