@@ -56,8 +56,9 @@ def oracle_fan(ctx, pos, n, gate, skp_band=None, nf=None):
     return fan, fine, skp
 
 
-def oracle_frame_scene(ctx, cam, W, H, disc=None, sky=None, stars=None, relativistic=False, a=0.0, volume=None):
-    c32, c64 = api.ks_cam_params(cam, M, a)
+def oracle_frame_scene(ctx, cam, W, H, disc=None, sky=None, stars=None, relativistic=False, a=0.0, volume=None,
+                       fisheye_deg=0.0):
+    c32, c64 = api.ks_cam_params(cam, M, a, fisheye_deg=fisheye_deg)
     dd = None
     if disc is not None:
         bb = disc.blackbody
@@ -153,10 +154,10 @@ def test_fan_reuse_and_speculation_leave_the_fans_unchanged(hip):
     ctx.close()
 
 
-@pytest.mark.parametrize("stars,relativistic", [(False, False), (True, True)])
-def test_composite_matches_oracle_on_the_same_fan(hip, stars, relativistic):
+@pytest.mark.parametrize("stars,relativistic,fisheye", [(False, False, 0.0), (True, True, 0.0), (False, False, 150.0)])
+def test_composite_matches_oracle_on_the_same_fan(hip, stars, relativistic, fisheye):
     W, H = 160, 96
-    print(f"case: composite {W}x{H} stars={stars} relativistic={relativistic}", flush=True)
+    print(f"case: composite {W}x{H} stars={stars} relativistic={relativistic} fisheye={fisheye}", flush=True)
     ctx = context(W, H)
     sp = ctx.set_starfield(height=H) if stars else None
     cam = camera()
@@ -168,7 +169,8 @@ def test_composite_matches_oracle_on_the_same_fan(hip, stars, relativistic):
     lay[..., 3] = rng.uniform(0.2, 1.0, lay.shape[:2])
     layer = layered.Image(ctx, W // 2, H // 2)
     layer.upload(lay)
-    layered.render_layered(ctx, sky, cam, SCH, layer, trace_layer=False, relativistic=relativistic)
+    layered.render_layered(ctx, sky, cam, SCH, layer, trace_layer=False, relativistic=relativistic,
+                           fisheye_deg=fisheye)
     got = ctx.download().astype(np.float64)
     fan, fine, skp = sky.download()
     for o in (layer, sky):
@@ -177,7 +179,7 @@ def test_composite_matches_oracle_on_the_same_fan(hip, stars, relativistic):
     st = None
     if stars:
         st = (sp, oracle.blackbody_lut(10000.0, float(sp[11]), float(sp[12]), int(sp[13])))
-    o = oracle_frame_scene(ctx, cam, W, H, stars=st, relativistic=relativistic)
+    o = oracle_frame_scene(ctx, cam, W, H, stars=st, relativistic=relativistic, fisheye_deg=fisheye)
     want = o.sky_composite(fan.astype(np.float64), fine.astype(np.float64), skp.astype(np.float64), lay)
     rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-2)
     print(f"rel diff p50 {np.median(rel):.2e} p99.9 {np.percentile(rel, 99.9):.2e} max {rel.max():.2e}")
