@@ -4,7 +4,7 @@ An MLP 5 -> 256 -> 256 -> 256 -> 256 -> 5 (SiLU), BF16 weights and FP32
 accumulation on the MI355X's MFMA units, maps a static/ZAMO camera's
 (r, theta, a) and a ray's local angles (alpha, beta) to the ray's escape
 direction and fate.  It is trained on rays the RK4 kernel traces; RK4 is
-also its fallback and the judge of its error.
+also its fallback and the measure of its error.
 """
 from __future__ import annotations
 
