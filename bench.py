@@ -161,6 +161,27 @@ def bench_frame(args):
             ctx.download_into(out)
 
         w2 = timed(step_d2h, ctx.sync, max(3, args.steps // 2), 1)
+        # the same with the copy overlapped: each frame goes device -> staging ->
+        # pinned host buffer on a copy stream while the next frame traces
+        pinned = [api.PinnedFrame(W, H), api.PinnedFrame(W, H)]
+        nxt = [0]
+
+        def step_ov():
+            ctx.clear()
+            ctx.trace(c32, c64, sp, targs)
+            ctx.download_async(pinned[nxt[0] % 2])
+            nxt[0] += 1
+
+        def drain():
+            ctx.sync()
+            ctx.download_wait()
+
+        w3 = timed(step_ov, drain, max(3, args.steps // 2), 1)
+        legs["incl_d2h_overlapped"] = {"value": round(W * H / w3 / 1e6, 3), "unit": "Mrays/s",
+                                       "note": "trace + the frame copied to pinned host memory each step, the copy of "
+                                               "frame k overlapping the trace of frame k+1 (two pinned buffers)"}
+        for pf in pinned:
+            pf.close()
         legs["incl_d2h"] = {"value": round(W * H / w2 / 1e6, 3), "unit": "Mrays/s",
                             "note": "trace + 24.9 MB frame copied to host each step"}
         if prec == L.ST_PREC_FP32:

@@ -183,6 +183,18 @@ int st_trace_depth(st_ctx* ctx, const float cam_params[49], const double* cam_pa
 int st_download_depth(st_ctx* ctx, float* host_out, size_t nfloats);
 /* Copy the per-pixel records to host float[H][W][ST_REC_N]. */
 int st_download_records(st_ctx* ctx, float* host_rec);
+/* Pinned host memory for st_download_async (hipHostMalloc; free with
+ * st_host_free).  No counterpart in the reference. */
+int st_host_alloc(size_t nfloats, float** out);
+int st_host_free(float* host);
+/* Asynchronous _download_rgb: the frame as it stands after the calls already
+ * made on the context is copied on the device to a staging buffer, then to
+ * `host` (float[H][W][3], pinned from st_host_alloc for the copy to overlap)
+ * on a copy stream, so the next frame's trace proceeds while it transfers.
+ * The host buffer is complete after st_download_wait.  No counterpart in the
+ * reference (a serving loop's frame stream). */
+int st_download_async(st_ctx* ctx, float* host);
+int st_download_wait(st_ctx* ctx);
 /* Counters and device timings of the last trace call (synchronises).  A sky
  * fan update or composite replaces the timings and keeps the counters of the
  * trace before it. */

@@ -94,6 +94,10 @@ SIGNATURES = {
     "st_sync": (C.c_int, [C.c_void_p]),
     "st_download": (C.c_int, [C.c_void_p, _F]),
     "st_download_records": (C.c_int, [C.c_void_p, _F]),
+    "st_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(_F)]),
+    "st_host_free": (C.c_int, [_F]),
+    "st_download_async": (C.c_int, [C.c_void_p, _F]),
+    "st_download_wait": (C.c_int, [C.c_void_p]),
     "st_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "st_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(FrameDesc), _F, _D, _F]),
     "st_trace_batch": (C.c_int, [C.c_void_p, C.POINTER(TraceArgs), C.c_int, C.c_int]),

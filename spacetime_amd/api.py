@@ -426,6 +426,17 @@ class HipPreviewContext:
         L.check(L.lib().st_download(self._h, L.fptr(out)))
         return out
 
+    def download_async(self, frame: "PinnedFrame"):
+        """Start copying the frame to `frame` (pinned host memory) on the
+        context's copy stream; later traces overlap the transfer.  The data
+        is there after download_wait()."""
+        if (frame.width, frame.height) != (self.width, self.height):
+            raise ValueError("pinned frame size differs from the context's")
+        L.check(L.lib().st_download_async(self._h, frame.ptr))
+
+    def download_wait(self):
+        L.check(L.lib().st_download_wait(self._h))
+
     def depth(self, nbuckets: int) -> np.ndarray:
         """The depth-bucketed accumulation: float32 (H, W, 3 * nbuckets)."""
         out = np.empty((self.height, self.width, 3 * nbuckets), dtype=np.float32)
@@ -452,6 +463,30 @@ class HipPreviewContext:
 
 # reference name kept for callers written against the Metal backend
 MetalPreviewContext = HipPreviewContext
+
+
+class PinnedFrame:
+    """A (height, width, 3) float32 frame in pinned host memory
+    (st_host_alloc), the target of HipPreviewContext.download_async."""
+
+    def __init__(self, width: int, height: int):
+        self.width, self.height = int(width), int(height)
+        p = L._F()
+        L.check(L.lib().st_host_alloc(3 * self.width * self.height, C.byref(p)))
+        self.ptr = p
+        self.array = np.ctypeslib.as_array(p, shape=(self.height, self.width, 3))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            L.lib().st_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _check_size(ctx: HipPreviewContext, width: int, height: int):

@@ -209,6 +209,7 @@ int st_ctx_destroy(st_ctx* c) {
     if (!c) return ST_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream_dl) (void)hipStreamSynchronize(c->stream_dl);
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     free_dev(c->sky); free_dev(c->lut); free_dev(c->vol); free_dev(c->star_lut);
@@ -222,6 +223,10 @@ int st_ctx_destroy(st_ctx* c) {
     if (c->ev_band) (void)hipEventDestroy(c->ev_band);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    free_dev(c->stage);
+    if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
+    if (c->ev_dl) (void)hipEventDestroy(c->ev_dl);
+    if (c->stream_dl) (void)hipStreamDestroy(c->stream_dl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ST_OK;
@@ -496,6 +501,49 @@ int st_download(st_ctx* c, float* host) {
     ST_HIP(hipMemcpyAsync(host, c->out, sizeof(float) * 3 * (size_t)c->width * c->height, hipMemcpyDeviceToHost,
                           c->stream));
     ST_HIP(hipStreamSynchronize(c->stream));
+    return ST_OK;
+}
+
+int st_host_alloc(size_t nfloats, float** out) {
+    if (!out) return fail(ST_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    if (nfloats == 0) return fail(ST_ERR_INVALID_ARG, "nfloats must be > 0");
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sizeof(float) * nfloats, hipHostMallocDefault) != hipSuccess)
+        return fail(ST_ERR_OOM, "hipHostMalloc failed");
+    *out = static_cast<float*>(p);
+    return ST_OK;
+}
+
+int st_host_free(float* p) {
+    if (p) (void)hipHostFree(p);
+    return ST_OK;
+}
+
+int st_download_async(st_ctx* c, float* host) {
+    if (!c || !host) return fail(ST_ERR_INVALID_ARG, "null argument");
+    const size_t bytes = sizeof(float) * 3 * (size_t)c->width * c->height;
+    if (!c->stage) {
+        ST_HIP(hipSetDevice(c->device));
+        ST_HIP(hipMalloc(&c->stage, bytes));
+        ST_HIP(hipStreamCreateWithFlags(&c->stream_dl, hipStreamNonBlocking));
+        ST_HIP(hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming));
+        ST_HIP(hipEventCreateWithFlags(&c->ev_dl, hipEventDisableTiming));
+    }
+    // the staging copy is free once the previous transfer has left it
+    if (c->dl_pending) ST_HIP(hipStreamWaitEvent(c->stream, c->ev_dl, 0));
+    ST_HIP(hipMemcpyAsync(c->stage, c->out, bytes, hipMemcpyDeviceToDevice, c->stream));
+    ST_HIP(hipEventRecord(c->ev_stage, c->stream));
+    ST_HIP(hipStreamWaitEvent(c->stream_dl, c->ev_stage, 0));
+    ST_HIP(hipMemcpyAsync(host, c->stage, bytes, hipMemcpyDeviceToHost, c->stream_dl));
+    ST_HIP(hipEventRecord(c->ev_dl, c->stream_dl));
+    c->dl_pending = true;
+    return ST_OK;
+}
+
+int st_download_wait(st_ctx* c) {
+    if (!c) return fail(ST_ERR_INVALID_ARG, "null context");
+    if (c->dl_pending) ST_HIP(hipEventSynchronize(c->ev_dl));
     return ST_OK;
 }
 

@@ -100,6 +100,13 @@ struct st_ctx {
     // traces the edge bands in FP32
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_band = nullptr, ev_join = nullptr;
+    // asynchronous frame download: a device staging copy of the frame and a
+    // copy stream (default priority), so the transfer of frame k overlaps
+    // the trace of frame k + 1
+    float* stage = nullptr;
+    hipStream_t stream_dl = nullptr;
+    hipEvent_t ev_stage = nullptr, ev_dl = nullptr;
+    bool dl_pending = false;
     bool last_mixed = false;
     bool have_last = false;
     bool last_counters = true;   // false after a call that counts nothing (a fluid step)

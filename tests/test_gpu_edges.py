@@ -107,3 +107,37 @@ def test_large_frame(hip, oracle_lib):
     rel = (np.abs(g[:, 5:8] - orec[:, 5:8]) / np.maximum(np.abs(orec[:, 5:8]), 1e-3)).max(axis=1)[same]
     print(f"fate agreement {agree:.4f}; within 1e-3: {np.mean(rel < 1e-3):.4f}; {st['ms_total']:.1f} ms")
     assert agree >= 0.99 and np.mean(rel < 1e-3) >= 0.97
+
+
+def test_async_download_to_pinned_frames(hip):
+    """Frames downloaded asynchronously into pinned buffers while the next
+    frame traces equal the synchronous downloads of the same frames."""
+    W, H = 160, 90
+    sc = scene(W, H)
+    print("case: async download 160x90", flush=True)
+    cams = [api.Camera(p, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), api.Lens(24.0))
+            for p in ((0.0, -30.0, 3.0), (5.0, -28.0, 2.0), (-4.0, -25.0, 6.0))]
+    ctx = api.HipPreviewContext(scenes.sky(), W, H, disc=sc.disc)
+    want = []
+    for cam in cams:
+        c32, c64 = api.ks_cam_params(cam, 1.0, 0.9)
+        ctx.clear()
+        ctx.trace(c32, c64, api.spacetime_params(sc.spacetime, cam, sc.rule), api.trace_args(W, H, sc.rule))
+        want.append(ctx.download())
+    frames = [api.PinnedFrame(W, H) for _ in cams]
+    for cam, f in zip(cams, frames):
+        c32, c64 = api.ks_cam_params(cam, 1.0, 0.9)
+        ctx.clear()
+        ctx.trace(c32, c64, api.spacetime_params(sc.spacetime, cam, sc.rule), api.trace_args(W, H, sc.rule))
+        ctx.download_async(f)
+    ctx.download_wait()
+    ctx.sync()
+    got = [f.array.copy() for f in frames]
+    with pytest.raises(ValueError):
+        ctx.download_async(api.PinnedFrame(8, 8))
+    for f in frames:
+        f.close()
+    ctx.close()
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+    assert not np.array_equal(got[0], got[1])
