@@ -113,7 +113,19 @@ __global__ __launch_bounds__(256) void k_trace_batch_once(const DeviceScene* __r
                                                           const int* tile_base, int nframes, int total_items,
                                                           int tw) {
     const DeviceScene& D = *Dp;
-    const unsigned item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
+    unsigned b = blockIdx.x;
+    if constexpr (VOL) {
+        // XCD-aware order for the gas march: the dispatcher deals blocks to the
+        // 8 XCDs round-robin (block b on XCD b % 8), so each XCD gets one
+        // contiguous run of wave tiles and its L2 sees one region's gas
+        // gathers (8 similar 4K orbit frames: 378 -> 368 ms per launch).  The
+        // thin-disc batch keeps the plain order: its frames differ in cost, and
+        // contiguous runs unbalance the XCDs (64 free-fall frames 28.6 ->
+        // 32.3 ms).
+        const unsigned full = gridDim.x & ~7u;
+        if (b < full) b = (b & 7u) * (full >> 3) + (b >> 3);
+    }
+    const unsigned item = __builtin_amdgcn_readfirstlane(b * 4u + (threadIdx.x >> 6));
     if (item >= (unsigned)total_items) return;
     bool traced;
     unsigned gas;
