@@ -204,14 +204,18 @@ thread_local std::string g_host_err;
 
 }  // namespace
 
+namespace st {
+int fail(int code, const std::string& msg);  // records the message st_last_error() returns
+}  // namespace st
+
 extern "C" {
 
 int st_camera_params_basis(const double pos[3], const double fwd[3], const double right[3], const double up[3],
                            double fov, const double beta[3], double M, double a, double out[49]) {
-    if (!pos || !fwd || !right || !up || !out) return ST_ERR_INVALID_ARG;
-    if (!(M > 0.0) || std::fabs(a) > M) return ST_ERR_INVALID_ARG;
+    if (!pos || !fwd || !right || !up || !out) return st::fail(ST_ERR_INVALID_ARG, "st_camera_params_basis: null pointer");
+    if (!(M > 0.0) || std::fabs(a) > M) return st::fail(ST_ERR_INVALID_ARG, "st_camera_params_basis: need M > 0 and |a| <= M");
     const V3 p = {pos[0], pos[1], pos[2]};
-    if (dot3(p, p) <= 0.0) return ST_ERR_INVALID_ARG;
+    if (dot3(p, p) <= 0.0) return st::fail(ST_ERR_INVALID_ARG, "st_camera_params_basis: camera at the origin");
     V4 u, Ef, Er, Eu;
     const V3 b = beta ? V3{beta[0], beta[1], beta[2]} : V3{0.0, 0.0, 0.0};
     tetrad(p, {fwd[0], fwd[1], fwd[2]}, {right[0], right[1], right[2]}, {up[0], up[1], up[2]}, M, a, b, u, Ef, Er, Eu);
@@ -222,8 +226,8 @@ int st_camera_params_basis(const double pos[3], const double fwd[3], const doubl
 int st_camera_params(const double pos[3], const double target[3], const double up[3], double fov,
                      const double velocity[3], double M, double a, double fisheye_deg, double focus_dist,
                      int equirect, double out[49]) {
-    if (!pos || !target || !up || !out) return ST_ERR_INVALID_ARG;
-    if (!(M > 0.0) || std::fabs(a) > M) return ST_ERR_INVALID_ARG;
+    if (!pos || !target || !up || !out) return st::fail(ST_ERR_INVALID_ARG, "st_camera_params: null pointer");
+    if (!(M > 0.0) || std::fabs(a) > M) return st::fail(ST_ERR_INVALID_ARG, "st_camera_params: need M > 0 and |a| <= M");
     const V3 p = {pos[0], pos[1], pos[2]};
     const V3 fwd = normalized({target[0] - pos[0], target[1] - pos[1], target[2] - pos[2]});
     const V3 right = normalized(cross(fwd, {up[0], up[1], up[2]}));
@@ -240,7 +244,8 @@ int st_camera_params(const double pos[3], const double target[3], const double u
 }
 
 int st_blackbody_lut(double wb, double tmin, double tmax, int n, float* out) {
-    if (!out || n < 1 || !(tmax > tmin) || !(wb > 0.0)) return ST_ERR_INVALID_ARG;
+    if (!out || n < 1 || !(tmax > tmin) || !(wb > 0.0))
+        return st::fail(ST_ERR_INVALID_ARG, "st_blackbody_lut: need out, n >= 1, tmax > tmin, wb > 0");
     double ref[3];
     bb_rgb(wb, ref);
     for (int i = 0; i < n; ++i) {
@@ -253,7 +258,7 @@ int st_blackbody_lut(double wb, double tmin, double tmax, int n, float* out) {
 }
 
 int st_make_sky(uint64_t seed, int w, int h, int nstars, double sigma, float* out) {
-    if (!out || w < 2 || h < 2 || nstars < 0) return ST_ERR_INVALID_ARG;
+    if (!out || w < 2 || h < 2 || nstars < 0) return st::fail(ST_ERR_INVALID_ARG, "st_make_sky: need out, w, h >= 2, nstars >= 0");
     // smooth gradient in theta: warmer near the +z pole, bluer near -z
     for (int y = 0; y < h; ++y) {
         const double th = M_PI * y / (double)(h - 1);
@@ -290,7 +295,7 @@ int st_make_sky(uint64_t seed, int w, int h, int nstars, double sigma, float* ou
 }
 
 int st_make_volume(uint64_t seed, int nr, int nphi, int nz, double sigma, float* out) {
-    if (!out || nr < 2 || nphi < 1 || nz < 2) return ST_ERR_INVALID_ARG;
+    if (!out || nr < 2 || nphi < 1 || nz < 2) return st::fail(ST_ERR_INVALID_ARG, "st_make_volume: need out, nr >= 2, nphi >= 1, nz >= 2");
     // Gaussian random field as a sum of random plane waves in (log s, phi, z)
     // index space, periodic in phi; exponentiated to a unit-mean log-normal.
     Rng rng(seed);
