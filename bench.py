@@ -661,7 +661,8 @@ def cpu_baseline(sc):
     except OSError:
         pass
     return {"value": round(rk4_mrays, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"full {sc.width}x{rows} frame, FP64 fixed-step RK4 (same algorithm), {t_rk4:.1f}s",
+            "sample": f"full {sc.width}x{rows} frame, FP64 fixed-step RK4 (same algorithm), {t_rk4:.1f}s wall "
+                      f"on {cores} threads ({t_rk4 * cores:.0f} thread-seconds)",
             "cpu_model": model, "nproc": os.cpu_count(),
             "one_core": {"value": round(sc.width * trows / t_1 / 1e6, 5),
                          "sample": f"FP64 RK4, {trows} centre rows x {sc.width}, 1 thread, {t_1:.1f}s"},
