@@ -418,3 +418,57 @@ def test_in_kernel_tsit5_matches_event_located_tsit5(oracle_lib, order, tol, dir
     if order == 46:
         r45 = oracle_scene(replace(sc, rule=replace(sc.rule, order=45, tol=tol))).trace_list(ij, "rk4")
         assert r[:, 1].mean() < r45[:, 1].mean()
+
+
+def _exact_schwarzschild_deflection(b, M=1.0):
+    """Total deflection of a light ray of impact parameter b > 3 sqrt(3) M:
+    alpha = 2 int_0^u0 du / sqrt(1/b^2 - u^2 + 2 M u^3) - pi, u0 the smallest
+    positive root (Darwin 1959), by quadrature after u = u0 (1 - t^2), which
+    removes the inverse-square-root endpoint singularity."""
+    from scipy.integrate import quad
+    roots = np.roots([2 * M, -1.0, 0.0, 1.0 / b ** 2])
+    u0 = min(r.real for r in roots if abs(r.imag) < 1e-12 and r.real > 0)
+
+    def f(t):
+        if t == 0.0:
+            return 2 * u0 / math.sqrt(u0 * (2 * u0 - 6 * M * u0 * u0))
+        g = 1.0 / b ** 2 - (u0 * (1 - t * t)) ** 2 + 2 * M * (u0 * (1 - t * t)) ** 3
+        return 2 * u0 * t / math.sqrt(g)
+
+    val, _ = quad(f, 0.0, 1.0, epsabs=1e-14, epsrel=1e-13, limit=200)
+    return 2 * val - math.pi
+
+
+@pytest.mark.parametrize("b", [5.4, 6.0, 8.0, 12.0])
+def test_schwarzschild_strong_deflection(oracle_lib, b):
+    """Strong field, down to b = 5.4 M (critical 5.196 M, deflection 2.92 rad):
+    the event-located Tsit5 against the exact deflection integral.  The ray
+    starts 4e4 M away along +y; the bending left outside the launch and
+    escape radii is about M b / R^2 < 1e-8."""
+    R = 4e4
+    sc = O.Scene(M=1.0, a=0.0, cam_pos=(0, -R, b), cam_target=(0, 0, b), cam_up=(0, 0, 1), fov=0.1,
+                 width=3, height=3, r_escape=2 * R, capture_mode=1, disc=None, esc_interp=False)
+    ray = sc.trace(1, 1, "tsit5", 1e-12)
+    assert ray.fate == 0
+    d = np.array(ray.dir)
+    got = math.atan2(-d[2], d[1])
+    assert got == pytest.approx(_exact_schwarzschild_deflection(b), abs=2e-8)
+
+
+def test_rk4_converges_at_fourth_order(oracle_lib):
+    """Fixed-step RK4 under the h = coef r rule: the escape-direction error
+    against Tsit5 (tol 1e-13) falls by 2^4 per halving of coef (b = 8 M,
+    launched at 1e3 M, escape at 1e5 M so the far-field tail is ~1e-9)."""
+    errs = []
+    truth = None
+    for coef in (0.2, 0.1, 0.05, 0.025):
+        sc = O.Scene(M=1.0, a=0.0, cam_pos=(0, -1e3, 8.0), cam_target=(0, 0, 8.0), cam_up=(0, 0, 1), fov=0.1,
+                     width=3, height=3, r_escape=1e5, capture_mode=1, disc=None, esc_interp=False, h_coef=coef,
+                     nmax=100000)
+        if truth is None:
+            truth = np.array(sc.trace(1, 1, "tsit5", 1e-13).dir)
+        ray = sc.trace(1, 1, "rk4")
+        assert ray.fate == 0
+        errs.append(np.linalg.norm(np.array(ray.dir) - truth))
+    orders = [math.log2(e1 / e2) for e1, e2 in zip(errs, errs[1:])]
+    assert all(3.8 < p < 4.4 for p in orders), orders
